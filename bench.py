@@ -1,0 +1,169 @@
+"""Benchmark: forward FP64 time stepping of the 2000 x 1000 colliding-thermals
+workload on one MI355X, in cell-steps/s, with the HBM-roofline fraction of
+the dominant kernel and the CPU oracle timed on the host as a baseline.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+
+A step is one evolve(): 2 directions x 3 RK stages over the 2000 x 1000
+interior (2e6 cell-steps), state resident in HBM. Prints one JSON line.
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = ("cell-steps/s fwd, batched, fwd+adjoint at FP64; % of HBM roofline, "
+          "1 MI355X")
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per interior cell per launch of the fused stage kernel:
+# stage 1 reads S once (init == forcing) and writes 32 B, stages 2 and 3 read
+# forcing + init and write out: (64 + 96 + 96) / 3 per launch on average.
+BYTES_PER_CELL_STEP = 512.0
+BYTES_PER_CELL_LAUNCH = BYTES_PER_CELL_STEP / 6.0
+
+
+def workload_params(nx=2000, nz=1000):
+    return {"nx": nx, "nz": nz, "xlen": 2.0e4, "zlen": 1.0e4, "hs": 2, "s": 4,
+            "ic_type": "collision", "cfl": 1.5, "max_speed": 450.0,
+            "dx": 2.0e4 / nx, "dz": 1.0e4 / nz,
+            "dt": min(2.0e4 / nx, 1.0e4 / nz) * 1.5 / 450.0}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(nx, nz, steps=1):
+    """The oracle's SciPy-convolve variant (the reference's own arithmetic
+    path) timed for `steps` timesteps of the same workload on 1 host core."""
+    import oracle as O
+    p = O.make_params(nx, nz, "collision", cfl=1.5, max_speed=450.0)
+    solver = O.OracleSolver(p, method="scipy")
+    t0 = time.perf_counter()
+    solver.run(steps)
+    el = time.perf_counter() - t0
+    return {"value": nx * nz * steps / el, "unit": "cell-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{steps} timestep(s) of collision {nx}x{nz} "
+                      f"({nx * nz * steps:.3g} cell-steps, {el:.1f} s) with the NumPy/"
+                      f"SciPy-convolve oracle, single thread; host {cpu_model()}, "
+                      f"os.cpu_count()={os.cpu_count()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=2000)
+    ap.add_argument("--nz", type=int, default=1000)
+    ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the x-slab halo-exchange design is described in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import pyminiweather_amd.solve as S
+    from pyminiweather_amd import _lib
+    from pyminiweather_amd._lib import ptr, stream_of
+    from pyminiweather_amd.data import initialize_fields
+    from pyminiweather_amd.ics import init
+    from pyminiweather_amd.mesh import MeshData
+    from pyminiweather_amd.post import compute_stats
+
+    torch.cuda.set_device(int(os.environ.get("LOCAL_RANK", "0")))
+    nx, nz = args.nx, args.nz
+    p = workload_params(nx, nz)
+    f = initialize_fields(p)
+    mesh = MeshData(p)
+    init(f, p, mesh)
+    S.reverse_direction = False
+    m0, e0 = compute_stats(p, f)
+
+    S.run(p, f, mesh, args.warmup)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    S.run(p, f, mesh, args.steps)
+    ev1.record()
+    torch.cuda.synchronize()
+    host_s = time.perf_counter() - t0
+    gpu_s = ev0.elapsed_time(ev1) / 1e3
+    cells = nx * nz
+    value = cells * args.steps / host_s
+
+    # per-launch kernel durations: HIP events around every launch
+    kp = max(1, min(args.profile_steps, args.steps))
+    kms = (ctypes.c_double * 2)()
+    flag = ctypes.c_int(1 if S.reverse_direction else 0)
+    _lib.call("pmw_evolve_timed", ptr(f.state), ptr(f.state_tmp), ptr(f.state_work),
+              ptr(f.hy_dens_cell), ptr(f.hy_dens_theta_cell), ptr(f.hy_dens_int),
+              ptr(f.hy_dens_theta_int), ptr(f.hy_pressure_int), None, 0, None, nx, nz,
+              float(p["dx"]), float(p["dz"]), float(p["dt"]),
+              S.hv_coeff(p["dx"], p["dt"]), S.hv_coeff(p["dz"], p["dt"]),
+              ctypes.byref(flag), kp, stream_of(), kms)
+    S.reverse_direction = bool(flag.value)
+    launches = 3 * kp  # per direction
+    names = {0: "k_stage_x (fused x RK stage)", 1: "k_stage_z (fused z RK stage)"}
+    kern = {}
+    for d in (0, 1):
+        avg_s = kms[d] / launches / 1e3
+        kern[names[d]] = {"avg_us": round(avg_s * 1e6, 3),
+                          "GBps": round(BYTES_PER_CELL_LAUNCH * cells / avg_s / 1e9, 1)}
+    dom = 0 if kms[0] >= kms[1] else 1
+    dom_avg_s = kms[dom] / launches / 1e3
+    achieved = BYTES_PER_CELL_LAUNCH * cells / dom_avg_s / 1e9
+    m1, e1 = compute_stats(p, f)
+    if not (abs(m1 - m0) <= 1e-12 * abs(m0)):
+        raise RuntimeError(f"mass not conserved: {m0} -> {m1}")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": host_s / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: colliding-thermals IC computed on the device",
+        "config": {"workload": f"collision {nx}x{nz} FP64 forward, RK3 + x/z splitting",
+                   "nx": nx, "nz": nz, "dt": p["dt"], "cfl": 1.5, "max_speed": 450.0},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": names[dom],
+                     "bytes_per_launch": BYTES_PER_CELL_LAUNCH * cells},
+        "step_roofline_frac": round(value * BYTES_PER_CELL_STEP / 1e9 / HBM_PEAK_GBS, 4),
+        "gpu_ms_per_step": gpu_s / args.steps * 1e3,
+        "kernels": kern,
+        "mass_rel_change": (m1 - m0) / m0,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(nx, nz)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,120 @@
+// pmw_common.h -- constants and pointwise device arithmetic shared by the
+// PyMiniWeather HIP kernels (gfx950).
+//
+// Every expression keeps the reference's floating-point operation order, and
+// the library is compiled with -ffp-contract=off, so that a face flux or a
+// cell update rounds exactly like the NumPy code it replaces. The only
+// expected differences from NumPy are the last-bit differences of pow/exp/
+// cos between the device math library and glibc.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pmw.h"
+
+namespace pmw {
+
+// pyminiweather/data/constants.py:13-27 (rd is 287 in a float Enum -> 287.0)
+constexpr double kHvBeta = 0.05;
+constexpr double kP0 = 1.0e5;
+constexpr double kC0 = 27.5629410929725921310572974482;
+constexpr double kGamma = 1.40027894002789400278940027894;
+constexpr double kGrav = 9.8;
+constexpr double kCp = 1004.0;
+constexpr double kCv = 717.0;
+constexpr double kRd = 287.0;
+constexpr double kPi = 3.14159265358979323846264338327;
+constexpr double kTheta0 = 300.0;
+constexpr double kExner0 = 1.0;
+
+// Convolution weights in the order the direct correlation applies them
+// (the stored kernels of data/fields.py:94-97, reversed).
+constexpr double kW4a = -1.0 / 12;  // taps 0 and 3 of the 4th-order kernel
+constexpr double kW4b = 7.0 / 12;   // taps 1 and 2
+// 3rd-difference kernel [1,-3,3,-1] reversed: [-1, 3, -3, 1]
+
+constexpr int kHs = 2;
+constexpr int kNv = 4;
+
+// Index helpers for a (4, nz+4, nx+4) state.
+struct Grid {
+  int nx, nz;
+  int ld;          // nx + 4
+  long long plane; // (nz + 4) * (nx + 4)
+  __host__ __device__ Grid(int nx_, int nz_)
+      : nx(nx_), nz(nz_), ld(nx_ + 4), plane((long long)(nz_ + 4) * (nx_ + 4)) {}
+  __host__ __device__ long long at(int v, int prow, int pcol) const {
+    return (long long)v * plane + (long long)prow * ld + pcol;
+  }
+};
+
+// 4-tap reconstruction at one face: value and 3rd difference, accumulated
+// left to right like scipy.signal.convolve's direct method.
+__device__ __forceinline__ void recon4(double s0, double s1, double s2,
+                                       double s3, double& val, double& d3) {
+  val = ((s0 * kW4a + s1 * kW4b) + s2 * kW4b) + s3 * kW4a;
+  d3 = ((s0 * -1.0 + s1 * 3.0) + s2 * -3.0) + s3 * 1.0;
+}
+
+// x-direction Euler flux with hyperviscosity (solve/interpolate.py:105-129).
+__device__ __forceinline__ void flux_x_face(const double val[4],
+                                            const double d3[4], double hyr,
+                                            double hyt, double hv,
+                                            double F[4]) {
+  const double rho = val[0] + hyr;
+  const double u = val[1] / rho;
+  const double w = val[2] / rho;
+  const double t = (val[3] + hyt) / rho;
+  const double p = kC0 * pow(rho * t, kGamma);
+  const double ru = rho * u;
+  F[0] = ru - hv * d3[0];
+  F[1] = (rho * (u * u) + p) - hv * d3[1];
+  F[2] = ru * w - hv * d3[2];
+  F[3] = ru * t - hv * d3[3];
+}
+
+// z-direction flux (solve/interpolate.py:153-186); `wall` marks faces k = 0
+// and k = nz, where w and the rho' 3rd difference are zeroed.
+__device__ __forceinline__ void flux_z_face(const double val[4], double d3[4],
+                                            double hyr, double hyt,
+                                            double hyp, double hv, bool wall,
+                                            double G[4]) {
+  const double rho = val[0] + hyr;
+  const double u = val[1] / rho;
+  double w = val[2] / rho;
+  const double t = (val[3] + hyt) / rho;
+  const double p = kC0 * pow(rho * t, kGamma) - hyp;
+  if (wall) {
+    w = 0.0;
+    d3[0] = 0.0;
+  }
+  const double rw = rho * w;
+  G[0] = rw - hv * d3[0];
+  G[1] = rw * u - hv * d3[1];
+  G[2] = (rho * (w * w) + p) - hv * d3[2];
+  G[3] = rw * t - hv * d3[3];
+}
+
+// Tendency from two face fluxes (solve/interpolate.py:209-215, 239-245).
+__device__ __forceinline__ double tend_of(double f_lo, double f_hi, double d) {
+  return (-(f_hi - f_lo)) / d;
+}
+
+// Value the reference's set_bc_z leaves in z-halo row `prow` (0, 1, nz+2 or
+// nz+3) of variable v, given the same column's value on the adjacent interior
+// edge row (bcs.py:92-148).
+__device__ __forceinline__ double wall_halo(int v, double edge_val,
+                                            double hy_edge, double hy_halo) {
+  if (v == 2) return 0.0;
+  if (v == 1) return edge_val / hy_edge * hy_halo;
+  return edge_val;
+}
+
+}  // namespace pmw
+
+#define PMW_CHECK_LAUNCH()                         \
+  do {                                             \
+    hipError_t e_ = hipGetLastError();             \
+    if (e_ != hipSuccess) return (int)e_;          \
+  } while (0)

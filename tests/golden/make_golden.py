@@ -140,6 +140,8 @@ def main(ref_root):
     save("evolve_gravity_40x20", **trajectory("gravity", 40, 20, {1, 10}, {1}))
     save("evolve_density-current_40x20",
          **trajectory("density-current", 40, 20, {1, 10}, {1}))
+    save("evolve_injection_40x20",
+         **trajectory("injection", 40, 20, {1, 10}, {1, 10}))
     save("evolve_thermal_100x50",
          **trajectory("thermal", 100, 50, {1, 2, 10, 1500}, {1}))
 

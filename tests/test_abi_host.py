@@ -1,0 +1,157 @@
+"""CPU-side checks of the C ABI and the host logic (no kernel is launched).
+
+* libpmw.so loads and exports every entry point include/pmw.h declares, and
+  the ctypes table declares each with the header's arity.
+* Argument errors come back as the documented codes before any launch.
+* Host-side numerics: numpy.linspace restatement, hv coefficient, geometry.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import REPO
+
+HEADER = os.path.join(REPO, "include", "pmw.h")
+
+
+def _header_decls():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    decls = {}
+    for m in re.finditer(r"\b(?:int|long long|const char\s*\*)\s+(pmw_\w+)\s*\(([^)]*)\)\s*;",
+                         text):
+        args = [a.strip() for a in m.group(2).split(",") if a.strip() and a.strip() != "void"]
+        decls[m.group(1)] = len(args)
+    return decls
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from pyminiweather_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        from pyminiweather_amd.build import build
+        build()
+    return _lib.load()
+
+
+def test_header_declares_entry_points():
+    decls = _header_decls()
+    for name in ("pmw_set_bc_x", "pmw_set_bc_z", "pmw_rk_stage", "pmw_evolve",
+                 "pmw_tendencies_x", "pmw_tendencies_z", "pmw_compute_stats", "pmw_init"):
+        assert name in decls
+
+
+def test_library_exports_every_declared_symbol(lib):
+    from pyminiweather_amd import _lib
+    decls = _header_decls()
+    assert set(decls) == set(_lib.SIGNATURES), "ctypes table and header disagree"
+    for name, nargs in decls.items():
+        fn = getattr(lib, name)          # raises AttributeError if not exported
+        assert len(fn.argtypes) == nargs, name
+
+
+def test_abi_version_and_error_text(lib):
+    assert lib.pmw_abi_version() == 100
+    assert b"invalid" in lib.pmw_error_string(-1)
+    assert b"alias" in lib.pmw_error_string(-2)
+    assert b"workspace" in lib.pmw_error_string(-3)
+
+
+FAKE = ctypes.c_void_p(0x10000)  # never dereferenced: every call below fails its checks
+
+
+def test_argument_errors_return_codes_without_launch(lib):
+    P2 = ctypes.c_void_p(0x20000)
+    P3 = ctypes.c_void_p(0x30000)
+    # sizes below the 4-point stencil's minimum
+    assert lib.pmw_rk_stage(0, FAKE, P2, P3, FAKE, FAKE, FAKE, FAKE, FAKE, None, 0, None,
+                            1, 10, 1.0, 0.1, -1.0, None) == -1
+    assert lib.pmw_set_bc_x(FAKE, 0, 10, 0, None, None, None, None) == -1
+    assert lib.pmw_set_bc_z(None, FAKE, 10, 10, None) == -1
+    # bad direction / bc enum
+    assert lib.pmw_rk_stage(2, FAKE, P2, P3, FAKE, FAKE, FAKE, FAKE, FAKE, None, 0, None,
+                            10, 10, 1.0, 0.1, -1.0, None) == -1
+    assert lib.pmw_rk_stage(0, FAKE, P2, P3, FAKE, FAKE, FAKE, FAKE, FAKE, None, 7, None,
+                            10, 10, 1.0, 0.1, -1.0, None) == -1
+    # injection without its row mask
+    assert lib.pmw_set_bc_x(FAKE, 10, 10, 1, None, FAKE, FAKE, None) == -1
+    # out aliasing forcing
+    assert lib.pmw_rk_stage(0, FAKE, P2, P2, FAKE, FAKE, FAKE, FAKE, FAKE, None, 0, None,
+                            10, 10, 1.0, 0.1, -1.0, None) == -2
+    flag = ctypes.c_int(0)
+    assert lib.pmw_evolve(FAKE, FAKE, P3, FAKE, FAKE, FAKE, FAKE, FAKE, None, 0, None, 10,
+                          10, 1.0, 1.0, 0.1, -1.0, -1.0, ctypes.byref(flag), 1, None) == -2
+    assert flag.value == 0
+    # workspace too small
+    need = lib.pmw_stats_workspace_size(2000, 1000)
+    assert need > 0
+    assert lib.pmw_compute_stats(FAKE, FAKE, FAKE, 2000, 1000, 10.0, 10.0, P2, need - 1,
+                                 P3, None) == -3
+    # bad IC kind / too many bubbles
+    geo = (ctypes.c_double * 16)()
+    assert lib.pmw_init(9, FAKE, FAKE, FAKE, FAKE, FAKE, FAKE, None, 10, 10, geo, None, 0,
+                        None) == -1
+    assert lib.pmw_init(0, FAKE, FAKE, FAKE, FAKE, FAKE, FAKE, None, 10, 10, geo, geo, 5,
+                        None) == -1
+
+
+def test_linspace_restatement_is_bitwise_numpy():
+    from pyminiweather_amd.data import linspace
+    for start, stop, num in [(-400.0, 20400.0, 104), (-300.0, 10300.0, 54),
+                             (0.0, 5100.0, 51), (5.0, 20005.0, 2000), (-0.75, 10.25, 33)]:
+        a = linspace(start, stop, num, "cpu").numpy()
+        b = np.linspace(start, stop, num, endpoint=False)
+        np.testing.assert_array_equal(a, b)
+
+
+def test_mesh_geometry_matches_numpy_meshes():
+    import oracle as O
+    from pyminiweather_amd.mesh import MeshData
+    p = O.make_params(100, 50, cfl=1.5, max_speed=450.0)
+    m = MeshData(p, device="cpu")
+    g = m.linspace_geometry()
+    assert g[:4] == [p["xlen"], p["zlen"], p["dx"], p["dz"]]
+    xe, ze = O.mesh_int_ext(p)
+    np.testing.assert_array_equal(np.arange(104) * g[5] + g[4], xe[0])
+    np.testing.assert_array_equal(np.arange(54) * g[7] + g[6], ze[:, 0])
+    np.testing.assert_array_equal(np.arange(54) * g[9] + g[8],
+                                  O.mesh_vertical_cell_centers_int_ext(p))
+    np.testing.assert_array_equal(np.arange(51) * g[11] + g[10], O.mesh_vertical_cell_edges(p))
+    xc, zc = O.mesh_cell_centers(p)
+    np.testing.assert_array_equal(np.arange(100) * g[13] + g[12], xc[0])
+    np.testing.assert_array_equal(np.arange(50) * g[15] + g[14], zc[:, 0])
+    xx, zz = m.get_mesh_int_ext()
+    np.testing.assert_array_equal(xx.numpy(), xe)
+    np.testing.assert_array_equal(zz.numpy(), ze)
+
+
+def test_hv_coeff_matches_reference_expression():
+    from pyminiweather_amd.solve import hv_coeff
+    import oracle.numpy_ref as R
+    for d, dt in [(200.0, 0.6666666666666666), (10.0, 0.03333333333333333), (50.0, 0.37)]:
+        assert hv_coeff(d, dt) == R._hv_coeff(d, dt) == -0.05 * d / (16 * dt)
+
+
+def test_state_validation_rejects_bad_buffers():
+    from types import SimpleNamespace
+    from pyminiweather_amd.ics import check_state
+    f = SimpleNamespace(shape=(4, 9, 14), state=torch.zeros(4, 9, 14, dtype=torch.float64))
+    check_state(f, torch.zeros(4, 9, 14, dtype=torch.float64))
+    for bad in (torch.zeros(4, 9, 13, dtype=torch.float64),
+                torch.zeros(4, 9, 14, dtype=torch.float32),
+                torch.zeros(4, 14, 9, dtype=torch.float64).transpose(1, 2)):
+        with pytest.raises(ValueError):
+            check_state(f, bad)
+
+
+def test_product_package_never_imports_oracle():
+    pkg = os.path.join(REPO, "pyminiweather_amd")
+    for root, _, files in os.walk(pkg):
+        for fn in files:
+            if fn.endswith(".py"):
+                src = open(os.path.join(root, fn)).read()
+                assert not re.search(r"^\s*(from|import)\s+oracle\b", src, re.M), fn
