@@ -35,6 +35,25 @@ def workload_params(nx=2000, nz=1000):
             "dt": min(2.0e4 / nx, 1.0e4 / nz) * 1.5 / 450.0}
 
 
+PROFILE_SUMMARY = os.path.join(REPO, "profiles", "stage_kernels_collision_2000x1000.txt")
+
+
+def traffic_from_profile(kernel_prefix):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a plain-copy calibration),
+    or None if the summary has no line for it."""
+    try:
+        with open(PROFILE_SUMMARY) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch" \
+                        and parts[1].startswith(kernel_prefix):
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -72,6 +91,9 @@ def main():
     ap.add_argument("--nz", type=int, default=1000)
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 plain 512 MiB device copies (scales the rocprof "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -89,6 +111,15 @@ def main():
     from pyminiweather_amd.post import compute_stats
 
     torch.cuda.set_device(int(os.environ.get("LOCAL_RANK", "0")))
+    if args.copy_calibration:
+        n = 512 * 2 ** 20 // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_copy": n * 8, "copies": 5}))
+        return
     nx, nz = args.nx, args.nz
     p = workload_params(nx, nz)
     f = initialize_fields(p)
@@ -124,6 +155,7 @@ def main():
     S.reverse_direction = bool(flag.value)
     launches = 3 * kp  # per direction
     names = {0: "k_stage_x (fused x RK stage)", 1: "k_stage_z (fused z RK stage)"}
+    prefixes = {0: "k_stage_x", 1: "k_stage_z"}
     kern = {}
     for d in (0, 1):
         avg_s = kms[d] / launches / 1e3
@@ -153,7 +185,7 @@ def main():
                    "nx": nx, "nz": nz, "dt": p["dt"], "cfl": 1.5, "max_speed": 450.0},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": names[dom],
+                     "traffic": traffic_from_profile(prefixes[dom]), "kernel": names[dom],
                      "bytes_per_launch": BYTES_PER_CELL_LAUNCH * cells},
         "step_roofline_frac": round(value * BYTES_PER_CELL_STEP / 1e9 / HBM_PEAK_GBS, 4),
         "gpu_ms_per_step": gpu_s / args.steps * 1e3,
