@@ -162,13 +162,14 @@ int pmw_rk_stage(int direction, const double* state_init,
                  const int* inj_rows, int nx, int nz, double d,
                  double dt_stage, double hv_coeff, void* stream);
 
-/* Replaces evolve (pyminiweather/solve/step.py:85-143): one full timestep,
- * two directions x three RK stages, six fused launches. Direction order is
+/* Replaces evolve (pyminiweather/solve/step.py:85-143): `nsteps` timesteps,
+ * each two directions x three RK stages, one fused launch per direction
+ * (pmw_evolve_ex documents the kernel choice). Direction order is
  * Z then X when *reverse_direction == 0, X then Z otherwise; the flag is
- * flipped on return (step.py:143). `state_work` is a third state-shaped
- * buffer that takes the stage-1 output so no stage reads and writes the same
- * buffer; on return `state` and `state_tmp` hold exactly what the
- * reference's evolve leaves in them. */
+ * flipped after every step (step.py:143). `state_work` is a third
+ * state-shaped buffer (the first direction's result, or the stage-1 output)
+ * so that no launch reads and writes the same buffer; on return `state` and
+ * `state_tmp` hold exactly what the reference's evolve leaves in them. */
 int pmw_evolve(double* state, double* state_tmp, double* state_work,
                const double* hy_dens_cell, const double* hy_dens_theta_cell,
                const double* hy_dens_int, const double* hy_dens_theta_int,
@@ -177,20 +178,27 @@ int pmw_evolve(double* state, double* state_tmp, double* state_work,
                double dt, double hv_coeff_x, double hv_coeff_z,
                int* reverse_direction, int nsteps, void* stream);
 
-/* pmw_evolve with a HIP event recorded before and after each of its 6*nsteps
- * launches; blocks until they finish and adds each launch's duration to
- * kernel_ms[0] (x-stage launches) or kernel_ms[1] (z-stage launches), a HOST
- * array of 2 doubles. For measurement only (creates and destroys events).
- * No reference counterpart. */
-int pmw_evolve_timed(double* state, double* state_tmp, double* state_work,
-                     const double* hy_dens_cell,
-                     const double* hy_dens_theta_cell,
-                     const double* hy_dens_int, const double* hy_dens_theta_int,
-                     const double* hy_pressure_int, const double* src_w,
-                     int bc, const int* inj_rows, int nx, int nz, double dx,
-                     double dz, double dt, double hv_coeff_x,
-                     double hv_coeff_z, int* reverse_direction, int nsteps,
-                     void* stream, double* kernel_ms);
+/* pmw_evolve with explicit kernel choice and optional per-launch timing.
+ * `tuning` (HOST, 3 ints, NULL = defaults {0, 256, 16}):
+ *   [0] mode: 0 = one fused launch per direction (all three RK stages,
+ *       periodic x only; the injection BC always uses mode 1), 1 = one
+ *       launch per RK stage (the pmw_rk_stage kernels);
+ *   [1] fused x block size: 256 or 512 threads (tile = that minus 12 cells);
+ *   [2] fused z segment: 8, 16, 24 or 32 rows per lane.
+ * Both modes leave bit-identical `state` and `state_tmp`.
+ * `kernel_ms` (HOST, 2 doubles, or NULL): if given, a HIP event is recorded
+ * around every launch, the call blocks until they finish, and each launch's
+ * duration is added to kernel_ms[0] (x launches) or kernel_ms[1] (z). For
+ * measurement only (creates and destroys events). No reference counterpart
+ * beyond evolve itself. */
+int pmw_evolve_ex(double* state, double* state_tmp, double* state_work,
+                  const double* hy_dens_cell, const double* hy_dens_theta_cell,
+                  const double* hy_dens_int, const double* hy_dens_theta_int,
+                  const double* hy_pressure_int, const double* src_w, int bc,
+                  const int* inj_rows, int nx, int nz, double dx, double dz,
+                  double dt, double hv_coeff_x, double hv_coeff_z,
+                  int* reverse_direction, int nsteps, const int* tuning,
+                  void* stream, double* kernel_ms);
 
 /* ---- reductions -------------------------------------------------------- */
 

@@ -37,9 +37,9 @@ SIGNATURES = {
                           _I, _I, _D, _D, _D, _P]),
     "pmw_evolve": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _I,
                         _D, _D, _D, _D, _D, _IP, _I, _P]),
-    "pmw_evolve_timed": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _I,
-                              _D, _D, _D, _D, _D, _IP, _I, _P,
-                              ctypes.POINTER(ctypes.c_double)]),
+    "pmw_evolve_ex": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _I,
+                           _D, _D, _D, _D, _D, _IP, _I, _IP, _P,
+                           ctypes.POINTER(ctypes.c_double)]),
     "pmw_stats_workspace_size": (_LL, [_I, _I]),
     "pmw_compute_stats": (_I, [_P, _P, _P, _I, _I, _D, _D, _P, _LL, _P, _P]),
     "pmw_init": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P, _I, _P]),

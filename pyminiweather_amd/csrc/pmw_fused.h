@@ -1,0 +1,30 @@
+// pmw_fused.h -- launch interface of the fused-direction kernels (internal).
+#pragma once
+
+#include "pmw_common.h"
+
+namespace pmw {
+
+// Arguments of one fused direction (three RK stages) of one grid.
+struct DirArgs {
+  const double* in;   // start state of the direction (interior read; RK init)
+  double* st;         // receives the stage-1 halo refresh (reference `state`)
+  double* out;        // result interior; must not alias `in`
+  double* tmp;        // stage-2 interior + stage-3 halo refresh (`state_tmp`)
+  const double* hy_cell;   // hy_dens_cell (x flux rows, z wall halos)
+  const double* hyt_cell;  // hy_dens_theta_cell (x flux rows)
+  const double* hyr_int;   // hy_dens_int (z faces)
+  const double* hyt_int;   // hy_dens_theta_int (z faces)
+  const double* hyp_int;   // hy_pressure_int (z faces)
+  const double* src_w;     // (nz, nx) w source or nullptr
+  double d;                // dx or dz
+  double dt1, dt2, dt3;    // dt/3, dt/2, dt
+  double hv;               // -hv_beta * d / (16 * dt_full)
+};
+
+bool fused_x_tile_ok(int nt);
+bool fused_z_seg_ok(int seg);
+int launch_dir_x(const DirArgs& a, Grid g, int nt, hipStream_t st);
+int launch_dir_z(const DirArgs& a, Grid g, int seg, hipStream_t st);
+
+}  // namespace pmw

@@ -1,0 +1,453 @@
+// pmw_fused.hip -- one launch per direction: the three RK stages of one
+// dimensional-splitting direction fused into a single kernel (gfx950).
+//
+// For one direction the reference runs (solve/step.py:111-141)
+//   S1 = S + dt/3 T(S),  S2 = S + dt/2 T(S1),  S3 = S + dt T(S2)
+// with a halo refresh of each forcing state. Each stage widens the stencil
+// by 2 cells, so an output tile needs S on the tile +- 6 cells: stage 1 is
+// computed on tile +- 4, stage 2 on tile +- 2, stage 3 on the tile. Per
+// interior cell and direction this reads S once and writes the result and
+// the stage-2 state (state_tmp): 96 B instead of the 256 B of three stage
+// launches. Every stage value is computed with the same operations as the
+// per-stage kernels, so results are bitwise identical to them.
+//
+// Buffers of a direction: `in` (its start state, read only), `out` (its
+// result interior, never aliases `in`), `st` (the reference's `state`: it
+// receives the halo refresh of stage 1, computed from `in`) and `tmp` (the
+// reference's `state_tmp`: stage-2 interior plus the stage-3 halo refresh).
+//
+//  * x (k_dir_x<NT>): one block of NT threads per (row, tile of NT-12 cells).
+//    S, the running stage state and face fluxes live in LDS; periodic wrap
+//    replaces the halo columns at every stage.
+//  * z (k_dir_z<L>): one lane per column marching a segment of L rows through
+//    a three-level register pipeline: at step j stage 1 takes S row j and
+//    finishes face j-1 and cell j-2, stage 2 finishes face j-3 and cell j-4,
+//    stage 3 face j-5 and cell j-6. Wall halo rows of each stage's forcing
+//    are synthesised in registers from that stage's edge row as it appears.
+//    No barriers. Extra lanes past nx write the corner columns of the halo
+//    rows. The bottom / top segment writes the z halo rows of st and tmp.
+// Block ids are remapped so that each XCD runs a contiguous range of tiles.
+#include "pmw_common.h"
+#include "pmw_fused.h"
+
+using namespace pmw;
+
+namespace {
+
+constexpr int kNumXcd = 8;
+
+__device__ __forceinline__ int xcd_logical(int b, int per_xcd) {
+  return (b % kNumXcd) * per_xcd + b / kNumXcd;
+}
+
+__device__ __forceinline__ int wrap_cell(int c, int nx) {
+  int w = c % nx;
+  return w < 0 ? w + nx : w;
+}
+
+// ------------------------------------------------------------------ x ------
+template <int NT>
+__global__ __launch_bounds__(NT) void k_dir_x(DirArgs a, Grid g, int ntiles, int nreal,
+                                              int per_xcd) {
+  constexpr int H = 6, C = NT - 2 * H;
+  __shared__ double s0[kNv][NT];  // S on slots 0..NT-1 (cell = i0 - 6 + slot)
+  __shared__ double s1[kNv][NT];  // S1, then S2 in place
+  __shared__ double fl[kNv][NT];  // face fluxes (face j between slots j-1, j)
+  const int logical = xcd_logical(blockIdx.x, per_xcd);
+  if (logical >= nreal) return;  // whole block, before any barrier
+  const int k = logical / ntiles;
+  const int i0 = (logical % ntiles) * C;
+  const int nx = g.nx, prow = k + kHs, t = threadIdx.x;
+  const int cell = i0 - H + t;
+  const int cw = wrap_cell(cell, nx);
+  const long long srow = (long long)k * nx;
+  {
+    double q[4];
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) q[v] = a.in[g.at(v, prow, cw + kHs)];
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) s0[v][t] = q[v];
+    if (cell == -2 || cell == -1 || cell == nx || cell == nx + 1) {
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) a.st[g.at(v, prow, cell + kHs)] = q[v];
+    }
+  }
+  const double hyr = a.hy_cell[prow], hyt = a.hyt_cell[prow];
+  const double src = a.src_w ? a.src_w[srow + cw] : 0.0;
+  __syncthreads();
+  // ---- stage 1: faces 2..NT-2, cells 2..NT-3, S -> s1
+  {
+    const int j = t + 2;
+    if (j <= NT - 2) {
+      double v4[4], d3[4], F[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v)
+        recon4(s0[v][j - 2], s0[v][j - 1], s0[v][j], s0[v][j + 1], v4[v], d3[v]);
+      flux_x_face(v4, d3, hyr, hyt, a.hv, F);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) fl[v][j] = F[v];
+    }
+  }
+  __syncthreads();
+  if (t >= 2 && t <= NT - 3) {
+    double td[4];
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) td[v] = tend_of(fl[v][t], fl[v][t + 1], a.d);
+    if (a.src_w) td[2] = td[2] + src;
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) s1[v][t] = s0[v][t] + a.dt1 * td[v];
+  }
+  __syncthreads();
+  // ---- stage 2: faces 4..NT-4, cells 4..NT-5, s1 -> s1 (in place)
+  {
+    const int j = t + 4;
+    if (j <= NT - 4) {
+      double v4[4], d3[4], F[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v)
+        recon4(s1[v][j - 2], s1[v][j - 1], s1[v][j], s1[v][j + 1], v4[v], d3[v]);
+      flux_x_face(v4, d3, hyr, hyt, a.hv, F);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) fl[v][j] = F[v];
+    }
+  }
+  __syncthreads();
+  if (t >= 4 && t <= NT - 5) {
+    double q[4];
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) {
+      const double td = tend_of(fl[v][t], fl[v][t + 1], a.d);
+      q[v] = td;
+    }
+    if (a.src_w) q[2] = q[2] + src;
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) q[v] = s0[v][t] + a.dt2 * q[v];
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) s1[v][t] = q[v];
+    // state_tmp: own cells, plus its x halos (set_bc_x of stage 3's forcing)
+    const bool own = cell >= i0 && cell < i0 + C && cell < nx;
+    const bool halo = cell == -2 || cell == -1 || cell == nx || cell == nx + 1;
+    if (own || halo) {
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) a.tmp[g.at(v, prow, cell + kHs)] = q[v];
+    }
+  }
+  __syncthreads();
+  // ---- stage 3: faces 6..NT-6, cells 6..NT-7 -> out
+  {
+    const int j = t + 6;
+    if (j <= NT - 6) {
+      double v4[4], d3[4], F[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v)
+        recon4(s1[v][j - 2], s1[v][j - 1], s1[v][j], s1[v][j + 1], v4[v], d3[v]);
+      flux_x_face(v4, d3, hyr, hyt, a.hv, F);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) fl[v][j] = F[v];
+    }
+  }
+  __syncthreads();
+  if (t >= 6 && t <= NT - 7 && cell < nx) {
+    double td[4];
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) td[v] = tend_of(fl[v][t], fl[v][t + 1], a.d);
+    if (a.src_w) td[2] = td[2] + src;
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) a.out[g.at(v, prow, cell + kHs)] = s0[v][t] + a.dt3 * td[v];
+  }
+}
+
+// ------------------------------------------------------------------ z ------
+// Face k of a z stage from the window rows k-2..k+1 (w[0..3]).
+__device__ __forceinline__ void z_face(const double w[4][4], const DirArgs& a, int k,
+                                       int nz, double G[4]) {
+  double v4[4], d3[4];
+#pragma unroll
+  for (int v = 0; v < kNv; ++v) recon4(w[0][v], w[1][v], w[2][v], w[3][v], v4[v], d3[v]);
+  flux_z_face(v4, d3, a.hyr_int[k], a.hyt_int[k], a.hyp_int[k], a.hv, k == 0 || k == nz, G);
+}
+
+__device__ __forceinline__ void shift4(double w[4][4]) {
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) w[p][v] = w[p + 1][v];
+}
+
+// Halo row values of set_bc_z from an edge row e (padded row pe) into halo
+// padded row ph.
+__device__ __forceinline__ void wall_row(const double e[4], const double* hy, int pe,
+                                         int ph, double out[4]) {
+#pragma unroll
+  for (int v = 0; v < kNv; ++v) out[v] = wall_halo(v, e[v], hy[pe], hy[ph]);
+}
+
+// One step of one z stage for one lane. The window slot of input row
+// base + p (p = 0..3) is (PH + 1 + p) & 3 where PH = step & 3, so advancing a
+// row changes the phase instead of copying registers. Input row r arrives,
+// face r-1 and cell r-2 are finished. Returns true and the cell's new value
+// in `res` when a cell was finished.
+template <int PH, int stage>
+__device__ __forceinline__ bool z_stage_step(double w[4][4], double F[4], const DirArgs& a,
+                                             const Grid& g, int r, int lo,
+                                             int hi, int pc, int i, const double* lds_row,
+                                             double dts, double res[4]) {
+  constexpr int S0 = (PH + 1) & 3, S1 = (PH + 2) & 3, S2 = (PH + 3) & 3, S3 = PH & 3;
+  const int nz = g.nz;
+  const double* hy = a.hy_cell;
+  // the new row is formed in nv and stored once, so that every window slot
+  // is written through a static index (no merged private-array stores)
+  if (r >= lo - 2 && r <= hi + 1 && r < nz + 2) {
+    double nv[4];
+    if (r >= 0 && r < nz) {
+      if (stage == 0) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) nv[v] = a.in[g.at(v, r + kHs, pc)];
+      } else {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) nv[v] = lds_row[v * 64];
+      }
+    } else if (r == nz) {
+      wall_row(w[S2], hy, nz + 1, nz + 2, nv);
+    } else {  // r == nz + 1, or r < 0 (rows -2, -1: refreshed when row 0 arrives)
+      wall_row(w[S1], hy, nz + 1, nz + 3, nv);
+    }
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) w[S3][v] = nv[v];
+    if (r == 0) {
+      double h1[4], h2[4];
+      wall_row(nv, hy, kHs, 0, h1);
+      wall_row(nv, hy, kHs, 1, h2);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) {
+        w[S1][v] = h1[v];
+        w[S2][v] = h2[v];
+      }
+    }
+  }
+  const int f = r - 1;
+  bool done = false;
+  if (f >= lo && f <= hi) {
+    double v4[4], d3[4], G[4];
+#pragma unroll
+    for (int v = 0; v < kNv; ++v)
+      recon4(w[S0][v], w[S1][v], w[S2][v], w[S3][v], v4[v], d3[v]);
+    flux_z_face(v4, d3, a.hyr_int[f], a.hyt_int[f], a.hyp_int[f], a.hv, f == 0 || f == nz, G);
+    const int c = f - 1;  // window row 1
+    if (c >= lo) {
+      double td[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) td[v] = tend_of(F[v], G[v], a.d);
+      td[2] = td[2] - w[S1][0] * kGrav;
+      if (a.src_w) td[2] = td[2] + a.src_w[(long long)c * g.nx + i];
+      if (stage == 0) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) res[v] = w[S1][v] + dts * td[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) res[v] = a.in[g.at(v, c + kHs, pc)] + dts * td[v];
+      }
+      done = true;
+    }
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) F[v] = G[v];
+  }
+  return done;
+}
+
+__device__ __forceinline__ void store_row(double* buf, const Grid& g, int prow, int pc,
+                                          const double q[4]) {
+#pragma unroll
+  for (int v = 0; v < kNv; ++v) buf[g.at(v, prow, pc)] = q[v];
+}
+
+// Body of one step j for the wave of `stage` (0, 1, 2 = RK stage 1, 2, 3).
+template <int PH, int stage>
+__device__ __forceinline__ void z_block_step(double w[4][4], double F[4], const DirArgs& a,
+                                             const Grid& g, double (*ring)[2][kNv][64],
+                                             int j, int lo, int hi, int k0,
+                                             int k1, int pc, int i, int lane, bool active,
+                                             double dts) {
+  const int nz = g.nz;
+  const double* hy = a.hy_cell;
+  const int r = j - 3 * stage;  // input row of this stage at step j
+  if (active) {
+    // rows of S1 / S2 come from the previous stage's ring (written one step ago)
+    const double* lds_row = stage > 0 ? &ring[stage - 1][r & 1][0][lane] : nullptr;
+    double res[4];
+    const bool done = z_stage_step<PH, stage>(w, F, a, g, r, lo, hi, pc, i, lds_row, dts, res);
+    if (stage == 0 && r == 0 && k0 == 0) {  // set_bc_z of stage 1 into st
+      double h[4];
+      wall_row(w[PH & 3], hy, kHs, 0, h);
+      store_row(a.st, g, 0, pc, h);
+      wall_row(w[PH & 3], hy, kHs, 1, h);
+      store_row(a.st, g, 1, pc, h);
+    }
+    if (stage == 0 && r == nz - 1 && k1 == nz && r >= lo - 2) {
+      double h[4];
+      wall_row(w[PH & 3], hy, nz + 1, nz + 2, h);
+      store_row(a.st, g, nz + 2, pc, h);
+      wall_row(w[PH & 3], hy, nz + 1, nz + 3, h);
+      store_row(a.st, g, nz + 3, pc, h);
+    }
+    if (done) {
+      const int c = r - 2;
+      if (stage < 2) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) ring[stage][c & 1][v][lane] = res[v];
+      }
+      if (stage == 1) {
+        if (c >= k0 && c < k1) store_row(a.tmp, g, c + kHs, pc, res);
+        if (c == 0 && k0 == 0) {  // set_bc_z of stage 3 into tmp
+          double h[4];
+          wall_row(res, hy, kHs, 0, h);
+          store_row(a.tmp, g, 0, pc, h);
+          wall_row(res, hy, kHs, 1, h);
+          store_row(a.tmp, g, 1, pc, h);
+        }
+        if (c == nz - 1 && k1 == nz) {
+          double h[4];
+          wall_row(res, hy, nz + 1, nz + 2, h);
+          store_row(a.tmp, g, nz + 2, pc, h);
+          wall_row(res, hy, nz + 1, nz + 3, h);
+          store_row(a.tmp, g, nz + 3, pc, h);
+        }
+      }
+      if (stage == 2) store_row(a.out, g, c + kHs, pc, res);
+    }
+  }
+  __syncthreads();
+}
+
+// March of one stage's wave over steps j0..j1 (one barrier per step; all
+// three waves of the block run the same number of steps).
+template <int stage>
+__device__ __forceinline__ void z_march(const DirArgs& a, const Grid& g,
+                                        double (*ring)[2][kNv][64], int j0, int j1, int lo,
+                                        int hi, int k0, int k1, int pc, int i, int lane,
+                                        bool active, double dts) {
+  double w[4][4], F[4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) w[p][v] = 0.0;
+#pragma unroll
+  for (int v = 0; v < kNv; ++v) F[v] = 0.0;
+#pragma unroll 1
+  for (int j = j0; j <= j1; ++j) {
+    // slot p holds row r-3+p once the step's row has arrived in slot 3
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) w[p][v] = w[p + 1][v];
+    z_block_step<3, stage>(w, F, a, g, ring, j, lo, hi, k0, k1, pc, i, lane, active, dts);
+  }
+}
+
+// z direction: a block is 3 waves over the same 64 columns and segment of L
+// output rows; wave s runs RK stage s+1. Stage s+1 consumes the rows stage s
+// produced one step earlier through a 2-slot LDS ring, so it lags by 3 rows
+// (2 for the stencil, 1 for the hand-over); one barrier per step.
+template <int L>
+__global__ __launch_bounds__(192) void k_dir_z(DirArgs a, Grid g, int ncolblk, int nreal,
+                                               int per_xcd) {
+  __shared__ double ring[2][2][kNv][64];
+  const int logical = xcd_logical(blockIdx.x, per_xcd);
+  if (logical >= nreal) return;  // whole block, before any barrier
+  const int nx = g.nx, nz = g.nz;
+  const int seg = logical / ncolblk;
+  const int lane = threadIdx.x & 63;
+  // wave-uniform; readfirstlane makes that visible to the compiler so the
+  // stage's bounds and branches are scalar
+  const int stage = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = (logical % ncolblk) * 64 + lane;  // interior column
+  const int k0 = seg * L, k1 = min(k0 + L, nz);
+  const double* hy = a.hy_cell;
+  if (stage == 0 && i >= nx && i < nx + 4) {
+    // corner columns of the halo rows (padded 0, 1, nx+2, nx+3) of st and tmp
+    const int q = i - nx;
+    const int cc = q < 2 ? q : nx + q;
+    const int lo = kHs, hi = nz + kHs - 1;
+    for (int pass = 0; pass < 2; ++pass) {
+      double* buf = pass == 0 ? a.st : a.tmp;
+      if (k0 == 0) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) {
+          const double e = buf[g.at(v, lo, cc)];
+          buf[g.at(v, 0, cc)] = wall_halo(v, e, hy[lo], hy[0]);
+          buf[g.at(v, 1, cc)] = wall_halo(v, e, hy[lo], hy[1]);
+        }
+      }
+      if (k1 == nz) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) {
+          const double e = buf[g.at(v, hi, cc)];
+          buf[g.at(v, nz + 2, cc)] = wall_halo(v, e, hy[hi], hy[nz + 2]);
+          buf[g.at(v, nz + 3, cc)] = wall_halo(v, e, hy[hi], hy[nz + 3]);
+        }
+      }
+    }
+  }
+  const bool active = i < nx;
+  const int pc = i + kHs;
+  // output rows of each stage: [a1,b1) S1, [a2,b2) S2, [k0,k1) result
+  const int a1 = max(k0 - 4, 0), b1 = min(k1 + 4, nz);
+  const int a2 = max(k0 - 2, 0), b2 = min(k1 + 2, nz);
+  const int lo = stage == 0 ? a1 : stage == 1 ? a2 : k0;
+  const int hi = stage == 0 ? b1 : stage == 1 ? b2 : k1;
+  if (stage == 0)
+    z_march<0>(a, g, ring, a1 - 2, k1 + 7, lo, hi, k0, k1, pc, i, lane, active, a.dt1);
+  else if (stage == 1)
+    z_march<1>(a, g, ring, a1 - 2, k1 + 7, lo, hi, k0, k1, pc, i, lane, active, a.dt2);
+  else
+    z_march<2>(a, g, ring, a1 - 2, k1 + 7, lo, hi, k0, k1, pc, i, lane, active, a.dt3);
+}
+
+template <int NT>
+int launch_x(const DirArgs& a, Grid g, hipStream_t st) {
+  constexpr int C = NT - 12;
+  const int ntiles = (g.nx + C - 1) / C;
+  const long long nreal = (long long)ntiles * g.nz;
+  const int per = (int)((nreal + kNumXcd - 1) / kNumXcd);
+  k_dir_x<NT><<<per * kNumXcd, NT, 0, st>>>(a, g, ntiles, (int)nreal, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
+template <int L>
+int launch_z(const DirArgs& a, Grid g, hipStream_t st) {
+  const int ncolblk = (g.nx + 4 + 63) / 64;
+  const int nseg = (g.nz + L - 1) / L;
+  const long long nreal = (long long)ncolblk * nseg;
+  const int per = (int)((nreal + kNumXcd - 1) / kNumXcd);
+  k_dir_z<L><<<per * kNumXcd, 192, 0, st>>>(a, g, ncolblk, (int)nreal, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
+}  // namespace
+
+namespace pmw {
+
+bool fused_x_tile_ok(int nt) { return nt == 256 || nt == 512; }
+bool fused_z_seg_ok(int seg) { return seg == 8 || seg == 16 || seg == 24 || seg == 32; }
+
+int launch_dir_x(const DirArgs& a, Grid g, int nt, hipStream_t st) {
+  switch (nt) {
+    case 256: return launch_x<256>(a, g, st);
+    case 512: return launch_x<512>(a, g, st);
+    default: return PMW_EINVAL;
+  }
+}
+
+int launch_dir_z(const DirArgs& a, Grid g, int seg, hipStream_t st) {
+  switch (seg) {
+    case 8: return launch_z<8>(a, g, st);
+    case 16: return launch_z<16>(a, g, st);
+    case 24: return launch_z<24>(a, g, st);
+    case 32: return launch_z<32>(a, g, st);
+    default: return PMW_EINVAL;
+  }
+}
+
+}  // namespace pmw

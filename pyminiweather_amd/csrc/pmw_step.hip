@@ -16,6 +16,7 @@
 // No kernel reads a location another block of the same launch writes, which
 // is why `out` may not alias `forcing`.
 #include "pmw_common.h"
+#include "pmw_fused.h"
 
 using namespace pmw;
 
@@ -597,8 +598,27 @@ int pmw_rk_stage(int direction, const double* state_init,
 
 namespace {
 
-// Shared body of pmw_evolve / pmw_evolve_timed. With `ev` non-null, event
-// ev[2j] is recorded before and ev[2j+1] after launch j (6 per step).
+struct Tuning {
+  int mode;   // 0: one fused launch per direction; 1: one launch per RK stage
+  int x_nt;   // fused x block size (256 or 512 threads; tile = x_nt - 12 cells)
+  int z_seg;  // fused z rows per lane segment (8, 16, 24 or 32)
+};
+constexpr Tuning kDefaultTuning = {0, 256, 16};
+
+int tuning_from(const int* t, Tuning& out) {
+  out = kDefaultTuning;
+  if (!t) return PMW_OK;
+  out.mode = t[0];
+  out.x_nt = t[1];
+  out.z_seg = t[2];
+  if (out.mode != 0 && out.mode != 1) return PMW_EINVAL;
+  if (out.mode == 0 && (!fused_x_tile_ok(out.x_nt) || !fused_z_seg_ok(out.z_seg)))
+    return PMW_EINVAL;
+  return PMW_OK;
+}
+
+// Shared body of the evolve entry points. With `ev` non-null, event ev[2j]
+// is recorded before and ev[2j+1] after launch j, dirs[j] = its direction.
 int evolve_impl(double* state, double* state_tmp, double* state_work,
                 const double* hy_dens_cell, const double* hy_dens_theta_cell,
                 const double* hy_dens_int, const double* hy_dens_theta_int,
@@ -606,7 +626,7 @@ int evolve_impl(double* state, double* state_tmp, double* state_work,
                 const int* inj_rows, int nx, int nz, double dx, double dz,
                 double dt, double hv_coeff_x, double hv_coeff_z,
                 int* reverse_direction, int nsteps, hipStream_t st,
-                hipEvent_t* ev, int* dirs) {
+                const Tuning& tune, hipEvent_t* ev, int* dirs, int* nlaunch) {
   if (!grid_ok(nx, nz) || bad_ptr(state) || bad_ptr(state_tmp) ||
       bad_ptr(state_work) || bad_ptr(hy_dens_cell) || bad_ptr(hy_dens_theta_cell) ||
       bad_ptr(hy_dens_int) || bad_ptr(hy_dens_theta_int) ||
@@ -619,21 +639,51 @@ int evolve_impl(double* state, double* state_tmp, double* state_work,
   Grid g(nx, nz);
   // stage dt as the reference computes them: dt/3, dt/2, dt/1 (step.py:111-141)
   const double dts[3] = {dt / 3, dt / 2, dt / 1};
+  // the injection inflow is only implemented by the per-stage kernels
+  const bool fused = tune.mode == 0 && bc == PMW_BC_PERIODIC;
   int launch = 0;
+  auto rec = [&](int k, int dir) -> int {
+    if (!ev) return PMW_OK;
+    if (k == 0) dirs[launch] = dir;
+    hipError_t e = hipEventRecord(ev[2 * launch + k], st);
+    return e == hipSuccess ? PMW_OK : (int)e;
+  };
   for (int step = 0; step < nsteps; ++step) {
     const int first = *reverse_direction ? 0 : 1;  // 0 = x, 1 = z
     for (int pass = 0; pass < 2; ++pass) {
       const int dir = pass == 0 ? first : 1 - first;
+      if (fused) {
+        // pass 0: S -> W, pass 1: W -> S; halos of stage 1 always into S
+        DirArgs a;
+        a.in = pass == 0 ? state : state_work;
+        a.out = pass == 0 ? state_work : state;
+        a.st = state;
+        a.tmp = state_tmp;
+        a.hy_cell = hy_dens_cell;
+        a.hyt_cell = hy_dens_theta_cell;
+        a.hyr_int = hy_dens_int;
+        a.hyt_int = hy_dens_theta_int;
+        a.hyp_int = hy_pressure_int;
+        a.src_w = src_w;
+        a.d = dir == 0 ? dx : dz;
+        a.dt1 = dts[0];
+        a.dt2 = dts[1];
+        a.dt3 = dts[2];
+        a.hv = dir == 0 ? hv_coeff_x : hv_coeff_z;
+        int rc = rec(0, dir);
+        if (rc) return rc;
+        rc = dir == 0 ? launch_dir_x(a, g, tune.x_nt, st) : launch_dir_z(a, g, tune.z_seg, st);
+        if (rc) return rc;
+        if ((rc = rec(1, dir))) return rc;
+        ++launch;
+        continue;
+      }
       // stage 1: S -> W; stage 2: W -> T; stage 3: T -> S (init = S always)
       double* forc[3] = {state, state_work, state_tmp};
       double* outs[3] = {state_work, state_tmp, state};
-      for (int s = 0; s < 3; ++s, ++launch) {
-        if (ev) {
-          hipError_t e = hipEventRecord(ev[2 * launch], st);
-          if (e != hipSuccess) return (int)e;
-          dirs[launch] = dir;
-        }
-        int rc;
+      for (int s = 0; s < 3; ++s) {
+        int rc = rec(0, dir);
+        if (rc) return rc;
         if (dir == 0)
           rc = launch_stage_x(0, state, forc[s], outs[s], nullptr, hy_dens_cell,
                               hy_dens_theta_cell, src_w, bc, inj_rows, g, dx, dts[s],
@@ -643,14 +693,13 @@ int evolve_impl(double* state, double* state_tmp, double* state_work,
                               hy_dens_int, hy_dens_theta_int, hy_pressure_int, src_w,
                               g, dz, dts[s], hv_coeff_z, st);
         if (rc) return rc;
-        if (ev) {
-          hipError_t e = hipEventRecord(ev[2 * launch + 1], st);
-          if (e != hipSuccess) return (int)e;
-        }
+        if ((rc = rec(1, dir))) return rc;
+        ++launch;
       }
     }
     *reverse_direction = !*reverse_direction;
   }
+  if (nlaunch) *nlaunch = launch;
   return PMW_OK;
 }
 
@@ -666,22 +715,33 @@ int pmw_evolve(double* state, double* state_tmp, double* state_work,
   return evolve_impl(state, state_tmp, state_work, hy_dens_cell, hy_dens_theta_cell,
                      hy_dens_int, hy_dens_theta_int, hy_pressure_int, src_w, bc,
                      inj_rows, nx, nz, dx, dz, dt, hv_coeff_x, hv_coeff_z,
-                     reverse_direction, nsteps, (hipStream_t)stream, nullptr, nullptr);
+                     reverse_direction, nsteps, (hipStream_t)stream, kDefaultTuning,
+                     nullptr, nullptr, nullptr);
 }
 
-int pmw_evolve_timed(double* state, double* state_tmp, double* state_work,
-                     const double* hy_dens_cell, const double* hy_dens_theta_cell,
-                     const double* hy_dens_int, const double* hy_dens_theta_int,
-                     const double* hy_pressure_int, const double* src_w, int bc,
-                     const int* inj_rows, int nx, int nz, double dx, double dz,
-                     double dt, double hv_coeff_x, double hv_coeff_z,
-                     int* reverse_direction, int nsteps, void* stream,
-                     double* kernel_ms) {
-  if (!kernel_ms || nsteps < 0 || nsteps > 100000) return PMW_EINVAL;
-  const int nl = 6 * nsteps;
-  hipEvent_t* ev = new hipEvent_t[2 * (size_t)nl + 1];
+int pmw_evolve_ex(double* state, double* state_tmp, double* state_work,
+                  const double* hy_dens_cell, const double* hy_dens_theta_cell,
+                  const double* hy_dens_int, const double* hy_dens_theta_int,
+                  const double* hy_pressure_int, const double* src_w, int bc,
+                  const int* inj_rows, int nx, int nz, double dx, double dz,
+                  double dt, double hv_coeff_x, double hv_coeff_z,
+                  int* reverse_direction, int nsteps, const int* tuning,
+                  void* stream, double* kernel_ms) {
+  Tuning tune;
+  int rc = tuning_from(tuning, tune);
+  if (rc) return rc;
+  if (!kernel_ms)
+    return evolve_impl(state, state_tmp, state_work, hy_dens_cell, hy_dens_theta_cell,
+                       hy_dens_int, hy_dens_theta_int, hy_pressure_int, src_w, bc,
+                       inj_rows, nx, nz, dx, dz, dt, hv_coeff_x, hv_coeff_z,
+                       reverse_direction, nsteps, (hipStream_t)stream, tune, nullptr,
+                       nullptr, nullptr);
+  if (nsteps < 0 || nsteps > 100000) return PMW_EINVAL;
+  const int nl = 6 * nsteps;  // upper bound on launches
+  hipEvent_t* ev = new hipEvent_t[2 * (size_t)nl + 2];
   int* dirs = new int[(size_t)nl + 1];
-  int made = 0, rc = PMW_OK;
+  int made = 0, used = 0;
+  rc = PMW_OK;
   for (; made < 2 * nl; ++made) {
     hipError_t e = hipEventCreate(&ev[made]);
     if (e != hipSuccess) { rc = (int)e; break; }
@@ -690,12 +750,13 @@ int pmw_evolve_timed(double* state, double* state_tmp, double* state_work,
     rc = evolve_impl(state, state_tmp, state_work, hy_dens_cell, hy_dens_theta_cell,
                      hy_dens_int, hy_dens_theta_int, hy_pressure_int, src_w, bc,
                      inj_rows, nx, nz, dx, dz, dt, hv_coeff_x, hv_coeff_z,
-                     reverse_direction, nsteps, (hipStream_t)stream, ev, dirs);
+                     reverse_direction, nsteps, (hipStream_t)stream, tune, ev, dirs,
+                     &used);
   kernel_ms[0] = kernel_ms[1] = 0.0;
-  if (rc == PMW_OK && nl > 0) {
-    hipError_t e = hipEventSynchronize(ev[2 * nl - 1]);
+  if (rc == PMW_OK && used > 0) {
+    hipError_t e = hipEventSynchronize(ev[2 * used - 1]);
     if (e != hipSuccess) rc = (int)e;
-    for (int j = 0; rc == PMW_OK && j < nl; ++j) {
+    for (int j = 0; rc == PMW_OK && j < used; ++j) {
       float ms = 0.f;
       e = hipEventElapsedTime(&ms, ev[2 * j], ev[2 * j + 1]);
       if (e != hipSuccess) { rc = (int)e; break; }

@@ -179,18 +179,34 @@ def discrete_step(params, fields, mesh, state_init, state_forcing, state_out, dt
               fields.nx, fields.nz, float(dt), _st(fields))
 
 
+# Kernel choice for evolve: (mode, fused x block size, fused z segment rows).
+# mode 0 = one fused launch per direction, 1 = one launch per RK stage. The
+# per-grid entries are filled by pyminiweather_amd.autotune.
+DEFAULT_TUNING = (0, 256, 16)
+_tuning_table = {}
+
+
+def tuning_for(nx, nz):
+    return _tuning_table.get((nx, nz), DEFAULT_TUNING)
+
+
+def set_tuning(nx, nz, tuning):
+    _tuning_table[(nx, nz)] = tuple(int(v) for v in tuning)
+
+
 def _evolve_n(params, fields, dt, nsteps):
     global reverse_direction
     flag = ctypes.c_int(1 if reverse_direction else 0)
     ic_type = params["ic_type"]
-    _lib.call("pmw_evolve", ptr(fields.state), ptr(fields.state_tmp),
+    tune = (ctypes.c_int * 3)(*tuning_for(fields.nx, fields.nz))
+    _lib.call("pmw_evolve_ex", ptr(fields.state), ptr(fields.state_tmp),
               ptr(fields.state_work), ptr(fields.hy_dens_cell),
               ptr(fields.hy_dens_theta_cell), ptr(fields.hy_dens_int),
               ptr(fields.hy_dens_theta_int), ptr(fields.hy_pressure_int),
               _src(params, fields), bc_code(ic_type), _inj(fields, ic_type),
               fields.nx, fields.nz, float(params["dx"]), float(params["dz"]), float(dt),
               hv_coeff(params["dx"], params["dt"]), hv_coeff(params["dz"], params["dt"]),
-              ctypes.byref(flag), int(nsteps), _st(fields))
+              ctypes.byref(flag), int(nsteps), tune, _st(fields), None)
     reverse_direction = bool(flag.value)
 
 

@@ -324,6 +324,47 @@ def test_evolve_matches_reference(ic, nx, nz, steps):
     assert S.reverse_direction == bool(g["reverse_direction_end"])
 
 
+def _evolve_ex(p, f, nsteps, tuning, flag=0):
+    import ctypes
+    import pyminiweather_amd.solve as S
+    from pyminiweather_amd import _lib
+    from pyminiweather_amd._lib import ptr
+    fl = ctypes.c_int(flag)
+    tune = (ctypes.c_int * 3)(*tuning)
+    bc = 1 if p["ic_type"] == "injection" else 0
+    _lib.call("pmw_evolve_ex", ptr(f.state), ptr(f.state_tmp), ptr(f.state_work),
+              ptr(f.hy_dens_cell), ptr(f.hy_dens_theta_cell), ptr(f.hy_dens_int),
+              ptr(f.hy_dens_theta_int), ptr(f.hy_pressure_int),
+              ptr(f.src_w) if p["ic_type"] == "gravity" else None, bc,
+              ptr(f.inj_rows) if bc else None, f.nx, f.nz, p["dx"], p["dz"], p["dt"],
+              S.hv_coeff(p["dx"], p["dt"]), S.hv_coeff(p["dz"], p["dt"]), ctypes.byref(fl),
+              nsteps, tune, _lib.stream_of(), None)
+    return fl.value
+
+
+FUSED_TUNINGS = [(0, 256, 8), (0, 256, 16), (0, 512, 24), (0, 512, 32)]
+
+
+@pytest.mark.parametrize("ic,nx,nz", [("collision", 60, 30), ("gravity", 40, 20),
+                                      ("thermal", 12, 7), ("collision", 2, 2),
+                                      ("density-current", 257, 37),
+                                      ("collision", 500, 70)])
+@pytest.mark.parametrize("tuning", FUSED_TUNINGS)
+def test_fused_directions_bitwise_equal_per_stage(ic, nx, nz, tuning):
+    """One launch per direction (all three RK stages, temporal blocking) and
+    one launch per stage leave bit-identical state and state_tmp, halos and
+    corners included, for awkward sizes (tiles and segments not dividing the
+    grid, grids smaller than one tile or segment)."""
+    p = O.make_params(nx, nz, ic, cfl=1.5, max_speed=450.0)
+    fa, _ = gpu_fields(p)
+    fb, _ = gpu_fields(p)
+    ra = _evolve_ex(p, fa, 3, (1, 256, 16))
+    rb = _evolve_ex(p, fb, 3, tuning)
+    assert ra == rb == 1
+    np.testing.assert_array_equal(host(fa.state), host(fb.state))
+    np.testing.assert_array_equal(host(fa.state_tmp), host(fb.state_tmp))
+
+
 def test_run_equals_evolve_loop_bitwise():
     import pyminiweather_amd.solve as S
     p = O.make_params(60, 30, "collision", cfl=1.5, max_speed=450.0)
