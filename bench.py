@@ -86,6 +86,65 @@ def cpu_baseline(nx, nz, steps=1):
                       f"os.cpu_count()={os.cpu_count()}"}
 
 
+def bench_ensemble(nsteps, loop_steps, autotune):
+    """Batched form: the 256-member mixed-size thermal ensemble (seed 0),
+    `nsteps` timesteps as HIP-graph replays of 2-step graphs, against looping
+    single-grid solves over the same members for `loop_steps` steps."""
+    import torch
+    import pyminiweather_amd.solve as S
+    from pyminiweather_amd.ensemble import Ensemble, mixed_thermal_members
+    from pyminiweather_amd.autotune import tune_ensemble, tune_grid
+    params, bubbles = mixed_thermal_members(256, seed=0)
+    ens = Ensemble(params, bubbles)
+    if autotune:
+        tuning, _ = tune_ensemble(ens)
+        # the loop of single solves gets its own per-grid tuning
+        seen = set()
+        for p, f in zip(ens.params, ens.fields):
+            if (p["nx"], p["nz"]) not in seen:
+                seen.add((p["nx"], p["nz"]))
+                tune_grid(p, f)
+    m0 = [m for m, _ in ens.stats()]
+    ens.evolve(2)
+    graph = ens.capture(step_pairs=1)
+    graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nsteps // 2):
+        graph.replay()
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+    steps_done = 2 * (nsteps // 2)
+    rate = ens.cells * steps_done / t_batched
+    m1 = [m for m, _ in ens.stats()]
+    drift = max(abs(b - a) / a for a, b in zip(m0, m1))
+    # looping single solves (one library call per member, all steps at once)
+    saved = S.reverse_direction
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p, f, mesh in zip(ens.params, ens.fields, ens.meshes):
+        S.run(p, f, mesh, loop_steps)
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t0
+    S.reverse_direction = saved
+    loop_rate = ens.cells * loop_steps / t_loop
+    sizes = {}
+    for p in params:
+        key = f"{p['nx']}x{p['nz']}"
+        sizes[key] = sizes.get(key, 0) + 1
+    return {"workload": "256-member thermal ensemble, grids drawn from 200x100/400x200/"
+                        "800x400 (seed 0), per-member dt and bubble",
+            "members": len(ens), "members_per_grid": sizes, "cells": ens.cells,
+            "steps": steps_done, "value": rate, "unit": "member-cell-steps/s",
+            "ms_per_step": t_batched / steps_done * 1e3,
+            "loop_of_single_solves": {"value": loop_rate, "steps": loop_steps},
+            "speedup_vs_loop": rate / loop_rate,
+            "step_roofline_frac": round(rate * BYTES_PER_CELL_STEP / 1e9 / HBM_PEAK_GBS, 4),
+            "max_rel_mass_drift": drift,
+            "launch": "one persistent launch per direction and step, 2-step HIP graph",
+            "tuning": list(ens.tuning)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +154,10 @@ def main():
     ap.add_argument("--nz", type=int, default=1000)
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=300)
+    ap.add_argument("--ensemble-loop-steps", type=int, default=20)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 512 MiB device copies (scales the rocprof "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -131,6 +194,10 @@ def main():
     init(f, p, mesh)
     S.reverse_direction = False
     m0, e0 = compute_stats(p, f)
+    tune_table = None
+    if not args.no_autotune:
+        from pyminiweather_amd.autotune import tune_grid
+        _, tune_table = tune_grid(p, f)
 
     S.run(p, f, mesh, args.warmup)
     torch.cuda.synchronize()
@@ -191,7 +258,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic: colliding-thermals IC computed on the device",
         "config": {"workload": f"collision {nx}x{nz} FP64 forward, RK3 + x/z splitting",
-                   "nx": nx, "nz": nz, "dt": p["dt"], "cfl": 1.5, "max_speed": 450.0},
+                   "nx": nx, "nz": nz, "dt": p["dt"], "cfl": 1.5, "max_speed": 450.0,
+                   "tuning": list(S.tuning_for(nx, nz))},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profile(prefixes[dom]), "kernel": names[dom],
@@ -201,6 +269,12 @@ def main():
         "kernels": kern,
         "mass_rel_change": (m1 - m0) / m0,
     }
+    if tune_table:
+        out["autotune_ms"] = {"/".join(map(str, k)): [round(v[0], 3), round(v[1], 3)]
+                              for k, v in tune_table.items()}
+    if not args.no_ensemble:
+        out["batched"] = bench_ensemble(args.ensemble_steps, args.ensemble_loop_steps,
+                                        not args.no_autotune)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(nx, nz)
     print(json.dumps(out), flush=True)

@@ -183,8 +183,9 @@ int pmw_evolve(double* state, double* state_tmp, double* state_work,
  *   [0] mode: 0 = one fused launch per direction (all three RK stages,
  *       periodic x only; the injection BC always uses mode 1), 1 = one
  *       launch per RK stage (the pmw_rk_stage kernels);
- *   [1] fused x block size: 256 or 512 threads (tile = that minus 12 cells);
- *   [2] fused z segment: 8, 16, 24 or 32 rows per lane.
+ *   [1] fused x block size: 256, 448 or 512 threads (tile = that minus 12
+ *       cells);
+ *   [2] fused z segment: 8, 16, 24, 32, 48 or 64 output rows per block.
  * Both modes leave bit-identical `state` and `state_tmp`.
  * `kernel_ms` (HOST, 2 doubles, or NULL): if given, a HIP event is recorded
  * around every launch, the call blocks until they finish, and each launch's
@@ -199,6 +200,49 @@ int pmw_evolve_ex(double* state, double* state_tmp, double* state_work,
                   double dt, double hv_coeff_x, double hv_coeff_z,
                   int* reverse_direction, int nsteps, const int* tuning,
                   void* stream, double* kernel_ms);
+
+/* ---- ensembles (batched, uneven grids) --------------------------------- */
+
+/* One member of an ensemble: its own Fields-like buffers and grid. All
+ * pointers are device pointers; src_w may be NULL. No reference counterpart
+ * (the reference advances one grid per process). */
+typedef struct pmw_member {
+  double* state;
+  double* state_tmp;
+  double* state_work;
+  const double* hy_dens_cell;
+  const double* hy_dens_theta_cell;
+  const double* hy_dens_int;
+  const double* hy_dens_theta_int;
+  const double* hy_pressure_int;
+  const double* src_w;
+  int nx;
+  int nz;
+  double dx;
+  double dz;
+  double dt;       /* this member's timestep: stages use dt/3, dt/2, dt */
+  double dt_full;  /* this member's params["dt"]: the hyperviscosity timestep */
+} pmw_member;
+
+/* Bytes of device workspace an ensemble of n members needs. */
+long long pmw_ensemble_workspace_size(int nmembers);
+
+/* Uploads the member descriptors and work-list offsets for `tuning` (same
+ * format as pmw_evolve_ex; mode must be 0) into `workspace` and waits for the
+ * copy. Call again whenever members or tuning change. */
+int pmw_ensemble_prepare(const pmw_member* members, int nmembers, const int* tuning,
+                         void* workspace, long long workspace_bytes, void* stream);
+
+/* evolve (pyminiweather/solve/step.py:85-143) for every member at once:
+ * `nsteps` timesteps, each one persistent launch per direction covering all
+ * members' tiles (the three RK stages fused as in pmw_evolve_ex). `members`
+ * and `tuning` must be what was prepared. Only kernel launches are issued,
+ * so a call can be captured in a HIP graph. Each member ends bit-identical
+ * to pmw_evolve on that member alone. kernel_ms as in pmw_evolve_ex (each
+ * launch is synchronised when given). */
+int pmw_ensemble_evolve(const pmw_member* members, int nmembers, const int* tuning,
+                        const void* workspace, int* reverse_direction, int nsteps,
+                        void* stream, double* kernel_ms);
 
 /* ---- reductions -------------------------------------------------------- */
 

@@ -17,6 +17,18 @@ _D = ctypes.c_double
 _LL = ctypes.c_longlong
 _IP = ctypes.POINTER(ctypes.c_int)
 
+class PmwMember(ctypes.Structure):
+    """struct pmw_member of include/pmw.h."""
+
+    _fields_ = [("state", _P), ("state_tmp", _P), ("state_work", _P),
+                ("hy_dens_cell", _P), ("hy_dens_theta_cell", _P), ("hy_dens_int", _P),
+                ("hy_dens_theta_int", _P), ("hy_pressure_int", _P), ("src_w", _P),
+                ("nx", _I), ("nz", _I), ("dx", _D), ("dz", _D), ("dt", _D),
+                ("dt_full", _D)]
+
+
+_MP = ctypes.POINTER(PmwMember)
+
 # name -> (restype, argtypes); must match include/pmw.h
 SIGNATURES = {
     "pmw_abi_version": (_I, []),
@@ -40,6 +52,10 @@ SIGNATURES = {
     "pmw_evolve_ex": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _I,
                            _D, _D, _D, _D, _D, _IP, _I, _IP, _P,
                            ctypes.POINTER(ctypes.c_double)]),
+    "pmw_ensemble_workspace_size": (_LL, [_I]),
+    "pmw_ensemble_prepare": (_I, [_MP, _I, _IP, _P, _LL, _P]),
+    "pmw_ensemble_evolve": (_I, [_MP, _I, _IP, _P, _IP, _I, _P,
+                                 ctypes.POINTER(ctypes.c_double)]),
     "pmw_stats_workspace_size": (_LL, [_I, _I]),
     "pmw_compute_stats": (_I, [_P, _P, _P, _I, _I, _D, _D, _P, _LL, _P, _P]),
     "pmw_init": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P, _I, _P]),

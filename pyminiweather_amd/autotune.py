@@ -1,0 +1,83 @@
+"""Kernel-shape autotuner for the fused time step.
+
+For one grid (or one ensemble) it times every candidate shape with HIP
+events around each launch (``pmw_evolve_ex`` / ``pmw_ensemble_evolve``
+``kernel_ms``) and keeps, independently, the fastest x block size and the
+fastest z segment length (the two directions are separate launches), then
+checks the fused choice against one launch per RK stage. Single grids are
+timed on scratch copies of the state, so tuning does not advance the
+solution; ensembles are timed in place (tuning advances them, like warm-up
+steps).
+"""
+import ctypes
+import itertools
+
+import torch
+
+from . import _lib
+from . import solve as S
+from ._lib import ptr, stream_of
+
+X_BLOCKS = (256, 448, 512)
+Z_SEGMENTS = (8, 16, 24, 32, 48, 64)
+
+
+def _time_grid(params, fields, tuning, steps, bufs):
+    kms = (ctypes.c_double * 2)()
+    flag = ctypes.c_int(0)
+    tune = (ctypes.c_int * 3)(*tuning)
+    bc = 1 if params["ic_type"] == "injection" else 0
+    _lib.call("pmw_evolve_ex", ptr(bufs[0]), ptr(bufs[1]), ptr(bufs[2]),
+              ptr(fields.hy_dens_cell), ptr(fields.hy_dens_theta_cell),
+              ptr(fields.hy_dens_int), ptr(fields.hy_dens_theta_int),
+              ptr(fields.hy_pressure_int),
+              ptr(fields.src_w) if params["ic_type"] == "gravity" else None, bc,
+              ptr(fields.inj_rows) if bc else None, fields.nx, fields.nz,
+              float(params["dx"]), float(params["dz"]), float(params["dt"]),
+              S.hv_coeff(params["dx"], params["dt"]), S.hv_coeff(params["dz"], params["dt"]),
+              ctypes.byref(flag), int(steps), tune, stream_of(fields.device), kms)
+    return kms[0], kms[1]
+
+
+def tune_grid(params, fields, steps=2, register=True):
+    """Pick (mode, x block, z segment) for this grid. Returns (tuning, table)
+    where table maps each timed tuning to its (x_ms, z_ms)."""
+    bufs = [fields.state.clone(), fields.state_tmp.clone(), fields.state_work.clone()]
+    table = {}
+    _time_grid(params, fields, S.DEFAULT_TUNING, 1, bufs)  # warm-up
+    for xb, zs in itertools.zip_longest(X_BLOCKS, Z_SEGMENTS):
+        t = (0, xb or X_BLOCKS[0], zs or Z_SEGMENTS[0])
+        table[t] = _time_grid(params, fields, t, steps, bufs)
+    best_x = min(X_BLOCKS, key=lambda xb: next(v[0] for t, v in table.items() if t[1] == xb))
+    best_z = min(Z_SEGMENTS, key=lambda zs: next(v[1] for t, v in table.items() if t[2] == zs))
+    fused = (0, best_x, best_z)
+    table[fused] = _time_grid(params, fields, fused, steps, bufs)
+    staged = (1, best_x, best_z)
+    table[staged] = _time_grid(params, fields, staged, steps, bufs)
+    best = fused if sum(table[fused]) <= sum(table[staged]) else staged
+    if register:
+        S.set_tuning(fields.nx, fields.nz, best)
+    del bufs
+    torch.cuda.synchronize(fields.device)
+    return best, table
+
+
+def tune_ensemble(ens, steps=2):
+    """Pick the fused (x block, z segment) for an Ensemble (timed in place)
+    and prepare it. Returns (tuning, table)."""
+    table = {}
+
+    def timed(t):
+        ens.set_tuning(t)
+        kms = (ctypes.c_double * 2)()
+        ens.evolve(steps, kernel_ms=kms)
+        return kms[0], kms[1]
+
+    for xb, zs in itertools.zip_longest(X_BLOCKS, Z_SEGMENTS):
+        t = (0, xb or X_BLOCKS[0], zs or Z_SEGMENTS[0])
+        table[t] = timed(t)
+    best_x = min(X_BLOCKS, key=lambda xb: next(v[0] for t, v in table.items() if t[1] == xb))
+    best_z = min(Z_SEGMENTS, key=lambda zs: next(v[1] for t, v in table.items() if t[2] == zs))
+    best = (0, best_x, best_z)
+    ens.set_tuning(best)
+    return best, table

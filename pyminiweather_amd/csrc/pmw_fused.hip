@@ -46,17 +46,25 @@ __device__ __forceinline__ int wrap_cell(int c, int nx) {
 }
 
 // ------------------------------------------------------------------ x ------
+// LDS of one x tile: S on slots 0..NT-1 (cell = i0 - 6 + slot), the running
+// stage state (S1, then S2 in place) and the face fluxes (face j between
+// slots j-1 and j).
 template <int NT>
-__global__ __launch_bounds__(NT) void k_dir_x(DirArgs a, Grid g, int ntiles, int nreal,
-                                              int per_xcd) {
+struct XTileLds {
+  double s0[kNv][NT];
+  double s1[kNv][NT];
+  double fl[kNv][NT];
+};
+
+// The three RK stages of one x tile: row k, output cells i0 .. i0+NT-13.
+// Called by all NT threads of the block (contains barriers).
+template <int NT>
+__device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int k, int i0,
+                                           XTileLds<NT>& sm) {
   constexpr int H = 6, C = NT - 2 * H;
-  __shared__ double s0[kNv][NT];  // S on slots 0..NT-1 (cell = i0 - 6 + slot)
-  __shared__ double s1[kNv][NT];  // S1, then S2 in place
-  __shared__ double fl[kNv][NT];  // face fluxes (face j between slots j-1, j)
-  const int logical = xcd_logical(blockIdx.x, per_xcd);
-  if (logical >= nreal) return;  // whole block, before any barrier
-  const int k = logical / ntiles;
-  const int i0 = (logical % ntiles) * C;
+  auto& s0 = sm.s0;
+  auto& s1 = sm.s1;
+  auto& fl = sm.fl;
   const int nx = g.nx, prow = k + kHs, t = threadIdx.x;
   const int cell = i0 - H + t;
   const int cw = wrap_cell(cell, nx);
@@ -155,6 +163,16 @@ __global__ __launch_bounds__(NT) void k_dir_x(DirArgs a, Grid g, int ntiles, int
 #pragma unroll
     for (int v = 0; v < kNv; ++v) a.out[g.at(v, prow, cell + kHs)] = s0[v][t] + a.dt3 * td[v];
   }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_dir_x(DirArgs a, Grid g, int ntiles, int nreal,
+                                              int per_xcd) {
+  constexpr int C = NT - 12;
+  __shared__ XTileLds<NT> sm;
+  const int logical = xcd_logical(blockIdx.x, per_xcd);
+  if (logical >= nreal) return;  // whole block, before any barrier
+  dir_x_tile<NT>(a, g, logical / ntiles, (logical % ntiles) * C, sm);
 }
 
 // ------------------------------------------------------------------ z ------
@@ -348,19 +366,17 @@ __device__ __forceinline__ void z_march(const DirArgs& a, const Grid& g,
 // output rows; wave s runs RK stage s+1. Stage s+1 consumes the rows stage s
 // produced one step earlier through a 2-slot LDS ring, so it lags by 3 rows
 // (2 for the stencil, 1 for the hand-over); one barrier per step.
+// One z block: 64 columns starting at column 64*colgrp, output rows of
+// segment `seg` (L rows). Called by all 192 threads (contains barriers).
 template <int L>
-__global__ __launch_bounds__(192) void k_dir_z(DirArgs a, Grid g, int ncolblk, int nreal,
-                                               int per_xcd) {
-  __shared__ double ring[2][2][kNv][64];
-  const int logical = xcd_logical(blockIdx.x, per_xcd);
-  if (logical >= nreal) return;  // whole block, before any barrier
+__device__ __forceinline__ void dir_z_block(const DirArgs& a, const Grid& g, int colgrp,
+                                            int seg, double (*ring)[2][kNv][64]) {
   const int nx = g.nx, nz = g.nz;
-  const int seg = logical / ncolblk;
   const int lane = threadIdx.x & 63;
   // wave-uniform; readfirstlane makes that visible to the compiler so the
   // stage's bounds and branches are scalar
   const int stage = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int i = (logical % ncolblk) * 64 + lane;  // interior column
+  const int i = colgrp * 64 + lane;  // interior column
   const int k0 = seg * L, k1 = min(k0 + L, nz);
   const double* hy = a.hy_cell;
   if (stage == 0 && i >= nx && i < nx + 4) {
@@ -403,6 +419,109 @@ __global__ __launch_bounds__(192) void k_dir_z(DirArgs a, Grid g, int ncolblk, i
     z_march<2>(a, g, ring, a1 - 2, k1 + 7, lo, hi, k0, k1, pc, i, lane, active, a.dt3);
 }
 
+template <int L>
+__global__ __launch_bounds__(192) void k_dir_z(DirArgs a, Grid g, int ncolblk, int nreal,
+                                               int per_xcd) {
+  __shared__ double ring[2][2][kNv][64];
+  const int logical = xcd_logical(blockIdx.x, per_xcd);
+  if (logical >= nreal) return;  // whole block, before any barrier
+  dir_z_block<L>(a, g, logical % ncolblk, logical / ncolblk, ring);
+}
+
+// ------------------------------------------------------------- ensemble --
+// Device descriptor of one member of an ensemble (uneven grids).
+struct MemberDev {
+  double* state;
+  double* tmp;
+  double* work;
+  const double* hyc;   // hy_dens_cell
+  const double* hytc;  // hy_dens_theta_cell
+  const double* hyri;  // hy_dens_int
+  const double* hyti;  // hy_dens_theta_int
+  const double* hypi;  // hy_pressure_int
+  const double* src;   // w source or nullptr
+  int nx, nz;
+  double dx, dz, dt1, dt2, dt3, hvx, hvz;
+  int x_start, z_start;  // first work item of this member in the x / z lists
+  int x_ntiles, z_ncol;  // x tiles per row, z column groups
+};
+
+__device__ __forceinline__ DirArgs member_args(const MemberDev& m, int dir, int pass) {
+  DirArgs a;
+  a.in = pass == 0 ? m.state : m.work;
+  a.out = pass == 0 ? m.work : m.state;
+  a.st = m.state;
+  a.tmp = m.tmp;
+  a.hy_cell = m.hyc;
+  a.hyt_cell = m.hytc;
+  a.hyr_int = m.hyri;
+  a.hyt_int = m.hyti;
+  a.hyp_int = m.hypi;
+  a.src_w = m.src;
+  a.d = dir == 0 ? m.dx : m.dz;
+  a.dt1 = m.dt1;
+  a.dt2 = m.dt2;
+  a.dt3 = m.dt3;
+  a.hv = dir == 0 ? m.hvx : m.hvz;
+  return a;
+}
+
+// Member owning work item `item` (starts are ascending); `mi` is a hint.
+template <int DIR>
+__device__ __forceinline__ int member_of(const MemberDev* m, int n, int item, int mi) {
+  auto start = [&](int q) { return DIR == 0 ? m[q].x_start : m[q].z_start; };
+  if (start(mi) > item) {  // binary search from scratch
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (start(mid) <= item) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+  }
+  while (mi + 1 < n && start(mi + 1) <= item) ++mi;
+  return mi;
+}
+
+// Persistent ensemble kernels: block b (XCD-remapped) takes the contiguous
+// chunk of work items [lb*chunk, (lb+1)*chunk), so neighbouring tiles of a
+// member share an XCD's L2. Every loop bound is block-uniform.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_ens_x(const MemberDev* __restrict__ m, int n, int pass,
+                                              int total, int chunk, int per_xcd) {
+  constexpr int C = NT - 12;
+  __shared__ XTileLds<NT> sm;
+  const int lb = xcd_logical(blockIdx.x, per_xcd);
+  const int first = lb * chunk;
+  const int last = min(total, first + chunk);
+  int mi = 0;
+  for (int item = first; item < last; ++item) {
+    mi = member_of<0>(m, n, item, mi);
+    const MemberDev& md = m[mi];
+    const DirArgs a = member_args(md, 0, pass);
+    const int local = item - md.x_start;
+    dir_x_tile<NT>(a, Grid(md.nx, md.nz), local / md.x_ntiles, (local % md.x_ntiles) * C, sm);
+    __syncthreads();  // the next tile rewrites the LDS this one last read
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(192) void k_ens_z(const MemberDev* __restrict__ m, int n, int pass,
+                                               int total, int chunk, int per_xcd) {
+  __shared__ double ring[2][2][kNv][64];
+  const int lb = xcd_logical(blockIdx.x, per_xcd);
+  const int first = lb * chunk;
+  const int last = min(total, first + chunk);
+  int mi = 0;
+  for (int item = first; item < last; ++item) {
+    mi = member_of<1>(m, n, item, mi);
+    const MemberDev& md = m[mi];
+    const DirArgs a = member_args(md, 1, pass);
+    const int local = item - md.z_start;
+    dir_z_block<L>(a, Grid(md.nx, md.nz), local % md.z_ncol, local / md.z_ncol, ring);
+    // dir_z_block ends on a barrier, so the ring is free for the next item
+  }
+}
+
 template <int NT>
 int launch_x(const DirArgs& a, Grid g, hipStream_t st) {
   constexpr int C = NT - 12;
@@ -429,12 +548,15 @@ int launch_z(const DirArgs& a, Grid g, hipStream_t st) {
 
 namespace pmw {
 
-bool fused_x_tile_ok(int nt) { return nt == 256 || nt == 512; }
-bool fused_z_seg_ok(int seg) { return seg == 8 || seg == 16 || seg == 24 || seg == 32; }
+bool fused_x_tile_ok(int nt) { return nt == 256 || nt == 448 || nt == 512; }
+bool fused_z_seg_ok(int seg) {
+  return seg == 8 || seg == 16 || seg == 24 || seg == 32 || seg == 48 || seg == 64;
+}
 
 int launch_dir_x(const DirArgs& a, Grid g, int nt, hipStream_t st) {
   switch (nt) {
     case 256: return launch_x<256>(a, g, st);
+    case 448: return launch_x<448>(a, g, st);
     case 512: return launch_x<512>(a, g, st);
     default: return PMW_EINVAL;
   }
@@ -446,8 +568,206 @@ int launch_dir_z(const DirArgs& a, Grid g, int seg, hipStream_t st) {
     case 16: return launch_z<16>(a, g, st);
     case 24: return launch_z<24>(a, g, st);
     case 32: return launch_z<32>(a, g, st);
+    case 48: return launch_z<48>(a, g, st);
+    case 64: return launch_z<64>(a, g, st);
     default: return PMW_EINVAL;
   }
 }
 
 }  // namespace pmw
+
+// ============================================================ ensemble ABI ==
+namespace {
+
+int ens_x_cells(int nt) { return nt - 12; }
+
+struct EnsPlan {
+  int x_total, z_total;
+};
+
+int plan_members(const pmw_member* mem, int n, int x_nt, int z_seg, MemberDev* dev,
+                 EnsPlan& plan) {
+  long long xs = 0, zs = 0;
+  for (int q = 0; q < n; ++q) {
+    const pmw_member& m = mem[q];
+    if (m.nx < 2 || m.nz < 2 || !m.state || !m.state_tmp || !m.state_work ||
+        !m.hy_dens_cell || !m.hy_dens_theta_cell || !m.hy_dens_int ||
+        !m.hy_dens_theta_int || !m.hy_pressure_int)
+      return PMW_EINVAL;
+    if (m.state == m.state_tmp || m.state == m.state_work || m.state_tmp == m.state_work)
+      return PMW_EALIAS;
+    const int xt = (m.nx + ens_x_cells(x_nt) - 1) / ens_x_cells(x_nt);
+    const int zc = (m.nx + 4 + 63) / 64;
+    const int zsg = (m.nz + z_seg - 1) / z_seg;
+    if (dev) {
+      MemberDev& d = dev[q];
+      d.state = m.state;
+      d.tmp = m.state_tmp;
+      d.work = m.state_work;
+      d.hyc = m.hy_dens_cell;
+      d.hytc = m.hy_dens_theta_cell;
+      d.hyri = m.hy_dens_int;
+      d.hyti = m.hy_dens_theta_int;
+      d.hypi = m.hy_pressure_int;
+      d.src = m.src_w;
+      d.nx = m.nx;
+      d.nz = m.nz;
+      d.dx = m.dx;
+      d.dz = m.dz;
+      d.dt1 = m.dt / 3;
+      d.dt2 = m.dt / 2;
+      d.dt3 = m.dt / 1;
+      d.hvx = -kHvBeta * m.dx / (16 * m.dt_full);
+      d.hvz = -kHvBeta * m.dz / (16 * m.dt_full);
+      d.x_start = (int)xs;
+      d.z_start = (int)zs;
+      d.x_ntiles = xt;
+      d.z_ncol = zc;
+    }
+    xs += (long long)xt * m.nz;
+    zs += (long long)zc * zsg;
+    if (xs > (1LL << 30) || zs > (1LL << 30)) return PMW_EINVAL;
+  }
+  plan.x_total = (int)xs;
+  plan.z_total = (int)zs;
+  return PMW_OK;
+}
+
+int ens_tuning(const int* t, int& x_nt, int& z_seg) {
+  x_nt = t ? t[1] : 256;
+  z_seg = t ? t[2] : 16;
+  if (t && t[0] != 0) return PMW_EINVAL;  // ensembles run the fused kernels only
+  if (!pmw::fused_x_tile_ok(x_nt) || !pmw::fused_z_seg_ok(z_seg)) return PMW_EINVAL;
+  return PMW_OK;
+}
+
+template <typename K>
+int persistent_grid(K kernel, int threads, int total, int& grid, int& chunk, int& per) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0);
+  if (e != hipSuccess) return (int)e;
+  long long want = (long long)ncu * (per_cu > 0 ? per_cu : 1);
+  if (want > total) want = total;
+  want = (want + kNumXcd - 1) / kNumXcd * kNumXcd;
+  per = (int)(want / kNumXcd);
+  grid = (int)want;
+  chunk = (int)((total + want - 1) / want);
+  return PMW_OK;
+}
+
+template <int NT>
+int launch_ens_x(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
+  int grid, chunk, per;
+  int rc = persistent_grid(k_ens_x<NT>, NT, total, grid, chunk, per);
+  if (rc) return rc;
+  k_ens_x<NT><<<grid, NT, 0, st>>>(d, n, pass, total, chunk, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
+template <int L>
+int launch_ens_z(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
+  int grid, chunk, per;
+  int rc = persistent_grid(k_ens_z<L>, 192, total, grid, chunk, per);
+  if (rc) return rc;
+  k_ens_z<L><<<grid, 192, 0, st>>>(d, n, pass, total, chunk, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
+int launch_ens(int dir, int x_nt, int z_seg, const MemberDev* d, int n, int pass,
+               const EnsPlan& plan, hipStream_t st) {
+  if (dir == 0) {
+    if (x_nt == 256) return launch_ens_x<256>(d, n, pass, plan.x_total, st);
+    if (x_nt == 448) return launch_ens_x<448>(d, n, pass, plan.x_total, st);
+    return launch_ens_x<512>(d, n, pass, plan.x_total, st);
+  }
+  switch (z_seg) {
+    case 8: return launch_ens_z<8>(d, n, pass, plan.z_total, st);
+    case 16: return launch_ens_z<16>(d, n, pass, plan.z_total, st);
+    case 24: return launch_ens_z<24>(d, n, pass, plan.z_total, st);
+    case 32: return launch_ens_z<32>(d, n, pass, plan.z_total, st);
+    case 48: return launch_ens_z<48>(d, n, pass, plan.z_total, st);
+    default: return launch_ens_z<64>(d, n, pass, plan.z_total, st);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+long long pmw_ensemble_workspace_size(int nmembers) {
+  if (nmembers < 1) return 0;
+  return (long long)sizeof(MemberDev) * nmembers;
+}
+
+int pmw_ensemble_prepare(const pmw_member* members, int nmembers, const int* tuning,
+                         void* workspace, long long workspace_bytes, void* stream) {
+  if (!members || nmembers < 1 || !workspace) return PMW_EINVAL;
+  if (workspace_bytes < pmw_ensemble_workspace_size(nmembers)) return PMW_ESPACE;
+  int x_nt, z_seg;
+  int rc = ens_tuning(tuning, x_nt, z_seg);
+  if (rc) return rc;
+  MemberDev* host = new MemberDev[nmembers];
+  EnsPlan plan;
+  rc = plan_members(members, nmembers, x_nt, z_seg, host, plan);
+  if (rc == PMW_OK) {
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(workspace, host, sizeof(MemberDev) * nmembers,
+                                  hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = (int)e;
+  }
+  delete[] host;
+  return rc;
+}
+
+int pmw_ensemble_evolve(const pmw_member* members, int nmembers, const int* tuning,
+                        const void* workspace, int* reverse_direction, int nsteps,
+                        void* stream, double* kernel_ms) {
+  if (!members || nmembers < 1 || !workspace || !reverse_direction || nsteps < 0 ||
+      nsteps > 100000)
+    return PMW_EINVAL;
+  int x_nt, z_seg;
+  int rc = ens_tuning(tuning, x_nt, z_seg);
+  if (rc) return rc;
+  EnsPlan plan;
+  rc = plan_members(members, nmembers, x_nt, z_seg, nullptr, plan);
+  if (rc) return rc;
+  const MemberDev* d = (const MemberDev*)workspace;
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  if (kernel_ms) {
+    kernel_ms[0] = kernel_ms[1] = 0.0;
+    if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess) {
+      if (ev[0]) hipEventDestroy(ev[0]);
+      return PMW_EINVAL;
+    }
+  }
+  for (int step = 0; step < nsteps && rc == PMW_OK; ++step) {
+    const int first = *reverse_direction ? 0 : 1;  // 0 = x, 1 = z
+    for (int pass = 0; pass < 2 && rc == PMW_OK; ++pass) {
+      const int dir = pass == 0 ? first : 1 - first;
+      if (kernel_ms) hipEventRecord(ev[0], st);
+      rc = launch_ens(dir, x_nt, z_seg, d, nmembers, pass, plan, st);
+      if (kernel_ms && rc == PMW_OK) {
+        hipEventRecord(ev[1], st);
+        float ms = 0.f;
+        if (hipEventSynchronize(ev[1]) == hipSuccess &&
+            hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess)
+          kernel_ms[dir] += ms;
+      }
+    }
+    if (rc == PMW_OK) *reverse_direction = !*reverse_direction;
+  }
+  if (kernel_ms) {
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+  }
+  return rc;
+}
+
+}  // extern "C"

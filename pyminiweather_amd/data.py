@@ -116,9 +116,11 @@ def linspace(start, stop, num, device):
     return torch.arange(num, dtype=torch.float64, device=device) * step + float(start)
 
 
-def initialize_fields(params, device=None):
+def initialize_fields(params, device=None, intermediates=True):
     """Allocate every array of ``Fields`` on the GPU, zero-filled
-    (pyminiweather/data/fields.py:58-123)."""
+    (pyminiweather/data/fields.py:58-123). With ``intermediates=False`` the
+    per-function scratch (flux, tend, vals_*, d3_vals_*) is left empty: the
+    fused time step never touches it (used for ensemble members)."""
     from . import _lib
 
     nx, nz, hs, s = params["nx"], params["nz"], params["hs"], params["s"]
@@ -131,6 +133,9 @@ def initialize_fields(params, device=None):
 
     def z(*shp):
         return torch.zeros(shp, dtype=torch.float64, device=device)
+
+    def zi(*shp):
+        return z(*shp) if intermediates else z(0)
 
     state = z(*shape)
     ic = params.get("ic_type", "thermal")
@@ -145,9 +150,9 @@ def initialize_fields(params, device=None):
     return Fields(
         nx=nx, nz=nz, hs=hs, nvariables=nv, s=s,
         state=state, state_tmp=state.clone(),
-        flux=z(nv, nz + 1, nx + 1), tend=z(nv, nz, nx),
-        vals_x=z(nv, nz, nx + 1), d3_vals_x=z(nv, nz, nx + 1),
-        vals_z=z(nv, nz + 1, nx), d3_vals_z=z(nv, nz + 1, nx),
+        flux=zi(nv, nz + 1, nx + 1), tend=zi(nv, nz, nx),
+        vals_x=zi(nv, nz, nx + 1), d3_vals_x=zi(nv, nz, nx + 1),
+        vals_z=zi(nv, nz + 1, nx), d3_vals_z=zi(nv, nz + 1, nx),
         hy_dens_cell=z(nz + 2 * hs), hy_dens_theta_cell=z(nz + 2 * hs),
         hy_dens_int=z(nz + 1), hy_pressure_int=z(nz + 1), hy_dens_theta_int=z(nz + 1),
         fourth_order_kernel=torch.tensor([-1.0 / 12, 7.0 / 12, 7.0 / 12, -1.0 / 12],

@@ -155,3 +155,15 @@ def test_product_package_never_imports_oracle():
             if fn.endswith(".py"):
                 src = open(os.path.join(root, fn)).read()
                 assert not re.search(r"^\s*(from|import)\s+oracle\b", src, re.M), fn
+
+
+def test_mixed_ensemble_draws_are_seeded_and_in_range():
+    from pyminiweather_amd.ensemble import GRID_CHOICES, mixed_thermal_members
+    p1, b1 = mixed_thermal_members(64, seed=0)
+    p2, b2 = mixed_thermal_members(64, seed=0)
+    assert p1 == p2 and b1 == b2
+    assert {(p["nx"], p["nz"]) for p in p1} == set(GRID_CHOICES)
+    for p, (amp, x0, z0, xr, zr) in zip(p1, b1):
+        assert 1.0 <= amp <= 5.0 and 6e3 <= x0 <= 14e3 and 1.5e3 <= z0 <= 3e3
+        assert 1.5e3 <= xr <= 2.5e3 and xr == zr
+        assert p["dt"] == min(p["dx"], p["dz"]) * 1.5 / 450.0
