@@ -244,6 +244,41 @@ int pmw_ensemble_evolve(const pmw_member* members, int nmembers, const int* tuni
                         const void* workspace, int* reverse_direction, int nsteps,
                         void* stream, double* kernel_ms);
 
+/* ---- adjoint (reverse mode) ------------------------------------------- */
+
+/* FP64 entries of workspace pmw_adjoint_step needs (5 state-sized buffers
+ * plus 8 per face). No reference counterpart. */
+long long pmw_adjoint_workspace_size(int nx, int nz);
+
+/* Adjoint of one RK stage of discrete_step (pyminiweather/solve/step.py:
+ * 21-82) in one direction (0 = x, 1 = z), excluding the init term:
+ * lam_out = scale * T'(state_forcing)^T lam_in + acc1 + acc2 + acc3, where
+ * T is halo + reconstruction + flux + tendency (+ gravity in z) and scale is
+ * the stage dt. Cotangent buffers have the state layout; only interiors are
+ * read and written. acc* may be NULL; lam_out may alias an acc but not
+ * lam_in. face_workspace: 8*(nz+1)*(nx+1) doubles. */
+int pmw_adjoint_stage(int direction, const double* state_forcing, const double* lam_in,
+                      double scale, const double* acc1, const double* acc2,
+                      const double* acc3, double* lam_out, double* face_workspace,
+                      const double* hy_dens_cell, const double* hy_dens_theta_cell,
+                      const double* hy_dens_int, const double* hy_dens_theta_int, int nx,
+                      int nz, double d, double hv_coeff, void* stream);
+
+/* Adjoint of one evolve step (pyminiweather/solve/step.py:85-143) taken
+ * from `state` with the given direction order (reverse_direction as the
+ * reference's toggle had it at that step): on entry lam is dL/d(state after
+ * the step), on return dL/d(state before it) (interiors). Recomputes the
+ * stage states from `state` (whose halos it rewrites) and reverses the two
+ * directions in the opposite order, each stage with pmw_adjoint_stage.
+ * Periodic x only. */
+int pmw_adjoint_step(double* state, double* lam, double* workspace,
+                     long long workspace_len, const double* hy_dens_cell,
+                     const double* hy_dens_theta_cell, const double* hy_dens_int,
+                     const double* hy_dens_theta_int, const double* hy_pressure_int,
+                     const double* src_w, int nx, int nz, double dx, double dz, double dt,
+                     double hv_coeff_x, double hv_coeff_z, int reverse_direction,
+                     void* stream);
+
 /* ---- reductions -------------------------------------------------------- */
 
 /* Number of FP64 workspace entries pmw_compute_stats needs for an nx x nz

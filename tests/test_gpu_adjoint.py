@@ -1,0 +1,199 @@
+"""Hand-written adjoint against FP64 torch autograd and finite differences."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(p, bubbles=None):
+    from pyminiweather_amd.data import initialize_fields
+    from pyminiweather_amd.ics import init
+    from pyminiweather_amd.mesh import MeshData
+    f = initialize_fields(p)
+    init(f, p, MeshData(p), bubbles=bubbles)
+    return f
+
+
+def _fdict(f):
+    return {k: getattr(f, k) for k in ("hy_dens_cell", "hy_dens_theta_cell", "hy_dens_int",
+                                       "hy_dens_theta_int", "hy_pressure_int")}
+
+
+def _perturbed_interior(f, seed):
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    s = f.state[:, 2:-2, 2:-2].cpu()
+    scale = torch.tensor([1e-3, 2.0, 2.0, 0.5], dtype=torch.float64)[:, None, None]
+    return (s + scale * torch.randn(s.shape, generator=g, dtype=torch.float64)).cuda()
+
+
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("ic,nx,nz", [("thermal", 12, 7), ("collision", 40, 20)])
+def test_stage_adjoint_matches_autograd(direction, ic, nx, nz):
+    """lam_f = a * T'(f)^T lam for one stage equals torch autograd of
+    <lam, a * T(f)> (halo, reconstruction, flux, tendency, gravity)."""
+    from oracle import torch_ref as R
+    from pyminiweather_amd import _lib
+    from pyminiweather_amd.solve import hv_coeff
+    p = O.make_params(nx, nz, ic, cfl=1.5, max_speed=450.0)
+    f = _fields(p)
+    u = _perturbed_interior(f, 11).requires_grad_(True)
+    lam_i = torch.randn(u.shape, dtype=torch.float64, device="cuda",
+                        generator=torch.Generator(device="cuda").manual_seed(3))
+    a = 0.7 * p["dt"]
+    d = p["dx"] if direction == 0 else p["dz"]
+    hv = hv_coeff(d, p["dt"])
+    tfun = R.tend_x if direction == 0 else R.tend_z
+    out = a * tfun(u, _fdict(f), d, hv)
+    (grad,) = torch.autograd.grad((out * lam_i).sum(), u)
+    fs = torch.zeros(f.shape, dtype=torch.float64, device="cuda")
+    fs[:, 2:-2, 2:-2] = u.detach()
+    lam = torch.zeros_like(fs)
+    lam[:, 2:-2, 2:-2] = lam_i
+    lout = torch.zeros_like(fs)
+    fw = torch.zeros(8 * (nz + 1) * (nx + 1), dtype=torch.float64, device="cuda")
+    _lib.call("pmw_adjoint_stage", direction, _lib.ptr(fs), _lib.ptr(lam), a, None, None,
+              None, _lib.ptr(lout), _lib.ptr(fw), _lib.ptr(f.hy_dens_cell),
+              _lib.ptr(f.hy_dens_theta_cell), _lib.ptr(f.hy_dens_int),
+              _lib.ptr(f.hy_dens_theta_int), nx, nz, d, hv, _lib.stream_of())
+    torch.cuda.synchronize()
+    got = lout[:, 2:-2, 2:-2].cpu().numpy()
+    ref = grad.cpu().numpy()
+    for v in range(4):
+        assert rel_l2(got[v], ref[v]) <= 1e-12, f"var {v}: {rel_l2(got[v], ref[v]):.3e}"
+
+
+@pytest.mark.parametrize("ic,nx,nz,steps", [("thermal", 40, 20, 6), ("gravity", 24, 12, 5)])
+def test_gradient_matches_torch_autograd(ic, nx, nz, steps):
+    """dL/dtheta' of L = 1/2 ||(rho*theta)'(T) - target||^2 over several full
+    timesteps (both direction orders, sqrt(N) checkpoints) equals FP64 torch
+    autograd through the restated step."""
+    from oracle import torch_ref as R
+    from pyminiweather_amd.adjoint import AdjointSolver
+    p = O.make_params(nx, nz, ic, cfl=1.5, max_speed=450.0)
+    f = _fields(p)
+    solver = AdjointSolver(p, f, steps)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    theta = 0.5 * torch.randn((nz, nx), dtype=torch.float64, device="cuda", generator=g)
+    target = 0.2 * torch.randn((nz, nx), dtype=torch.float64, device="cuda", generator=g)
+    loss, grad = solver.loss_and_grad(theta, target)
+
+    th = theta.clone().requires_grad_(True)
+    w = f.hy_dens_cell[2:nz + 2][:, None]
+    u0 = torch.zeros((4, nz, nx), dtype=torch.float64, device="cuda")
+    u0 = torch.cat([u0[:3], (w * th)[None]], dim=0)
+    src = f.src_w if ic == "gravity" else None
+    uT = R.evolve(u0, p, _fdict(f), steps, src_w=src)
+    resid = uT[3] - target
+    lref = 0.5 * torch.sum(resid * resid)
+    (gref,) = torch.autograd.grad(lref, th)
+    lref = lref.item()
+    assert abs(loss - lref) <= 1e-12 * abs(lref)
+    r = rel_l2(grad.cpu().numpy(), gref.cpu().numpy())
+    assert r <= 1e-11, f"gradient rel L2 {r:.3e}"
+
+
+def test_checkpoint_interval_does_not_change_gradient():
+    from pyminiweather_amd.adjoint import AdjointSolver
+    p = O.make_params(30, 16, "thermal", cfl=1.5, max_speed=450.0)
+    f = _fields(p)
+    g = torch.Generator(device="cuda").manual_seed(9)
+    theta = torch.randn((16, 30), dtype=torch.float64, device="cuda", generator=g)
+    target = torch.randn((16, 30), dtype=torch.float64, device="cuda", generator=g)
+    grads = []
+    for every in (1, 3, 9):   # 9 steps: all stored, sqrt(N), one segment
+        solver = AdjointSolver(p, f, 9, checkpoint_every=every)
+        grads.append(solver.loss_and_grad(theta, target)[1].cpu().numpy())
+    np.testing.assert_array_equal(grads[0], grads[1])
+    np.testing.assert_array_equal(grads[0], grads[2])
+
+
+def test_autograd_function_composes():
+    """EvolveFunction inside a torch graph gives torch.autograd the HIP
+    adjoint: the gradient of sum(final^2) w.r.t. the initial interior equals
+    the torch restatement's."""
+    from oracle import torch_ref as R
+    from pyminiweather_amd.adjoint import AdjointSolver, EvolveFunction
+    p = O.make_params(20, 10, "collision", cfl=1.5, max_speed=450.0)
+    f = _fields(p)
+    solver = AdjointSolver(p, f, 4)
+    u0 = _perturbed_interior(f, 2)
+    a = u0.clone().requires_grad_(True)
+    (ga,) = torch.autograd.grad((EvolveFunction.apply(a, solver) ** 2).sum(), a)
+    b = u0.clone().requires_grad_(True)
+    (gb,) = torch.autograd.grad((R.evolve(b, p, _fdict(f), 4) ** 2).sum(), b)
+    for v in range(4):
+        assert rel_l2(ga[v].cpu().numpy(), gb[v].cpu().numpy()) <= 1e-11
+
+
+def _config3():
+    """400x200 thermal, 400 RK3 steps; the target is the forward run from
+    the thermal IC plus i.i.d. N(0, 0.1 K) theta' noise drawn with numpy
+    default_rng(1). Returns (solver, theta0, target, loss, grad)."""
+    from pyminiweather_amd.adjoint import AdjointSolver
+    from pyminiweather_amd.ensemble import member_params
+    p = member_params(400, 200)
+    f = _fields(p)
+    solver = AdjointSolver(p, f, 400)
+    w = solver.row_weight()[:, None]
+    theta0 = f.state[3, 2:-2, 2:-2] / w          # the thermal IC as a theta' field
+    noise = torch.from_numpy(np.random.default_rng(1).normal(0.0, 0.1, (200, 400))).cuda()
+    target, _ = solver.forward(solver.state_from_theta(theta0 + noise))
+    target = target[3, 2:-2, 2:-2].clone()
+    loss, grad = solver.loss_and_grad(theta0, target)
+    return solver, theta0, target, loss, grad
+
+
+def _central_fd(solver, theta0, target, k, i, eps):
+    e = torch.zeros_like(theta0)
+    e[k, i] = eps
+    return (solver.loss(theta0 + e, target) - solver.loss(theta0 - e, target)) / (2 * eps)
+
+
+def test_config3_gradient_matches_central_differences():
+    """The adjoint gradient w.r.t. the initial theta' (sqrt(N) = 20-step
+    checkpoints) against central differences on 64 random cells.
+
+    At eps = 1e-6 the quotient is limited by the loss's own FP64 round-off:
+    every face rounds a ~1e5 Pa pressure at ~1e-11 Pa, and the perturbed and
+    unperturbed runs round differently everywhere the perturbation reaches,
+    so L carries ~1e-13 of noise and the quotient ~1e-13 / 2e-6. That noise
+    is measured here as the spread of L under 1e-13 perturbations (whose
+    linear effect, |g| * 1e-13, is ~1e-16, far below it), and each eps = 1e-6
+    quotient must agree with the adjoint within 6 sigma of it (plus 1e-6
+    relative). At eps = 1e-3, where that noise is 1000x smaller and the
+    truncation error of the central difference is still ~1e-10, the same
+    per-cell bound holds and the 64 quotients agree with the adjoint to 1e-6
+    relative in the L2 norm over the 64 cells."""
+    solver, theta0, target, loss, grad = _config3()
+    assert solver.every == 20
+    assert loss > 0
+    base = solver.loss(theta0, target)
+    assert base == loss
+    rng = np.random.default_rng(7)
+    cells = [(int(rng.integers(200)), int(rng.integers(400))) for _ in range(64)]
+    jitter = []
+    for k, i in cells[:8]:
+        e = torch.zeros_like(theta0)
+        e[k, i] = 1e-13
+        jitter.append(solver.loss(theta0 + e, target) - base)
+    sigma_l = float(np.sqrt(np.mean(np.square(jitter))))
+    assert 0.0 < sigma_l < 1e-10 * loss
+    g = np.array([float(grad[k, i]) for k, i in cells])
+    report = [f"sigma_L {sigma_l:.3e}"]
+    for eps in (1e-6, 1e-3):
+        sigma_fd = math.sqrt(2.0) * sigma_l / (2 * eps)
+        fd = np.array([_central_fd(solver, theta0, target, k, i, eps) for k, i in cells])
+        err = np.abs(fd - g)
+        bad = np.nonzero(err > 6 * sigma_fd + 1e-6 * np.abs(g))[0]
+        assert bad.size == 0, [(cells[j], fd[j], g[j], sigma_fd) for j in bad]
+        rel = float(np.linalg.norm(fd - g) / np.linalg.norm(g))
+        report.append(f"eps={eps:g}: rel L2 {rel:.3e}, noise {sigma_fd:.2e}")
+        if eps == 1e-3:
+            assert rel <= 1e-6, report
+    print("; ".join(report))
