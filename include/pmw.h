@@ -246,38 +246,54 @@ int pmw_ensemble_evolve(const pmw_member* members, int nmembers, const int* tuni
 
 /* ---- adjoint (reverse mode) ------------------------------------------- */
 
-/* FP64 entries of workspace pmw_adjoint_step needs (5 state-sized buffers
- * plus 8 per face). No reference counterpart. */
-long long pmw_adjoint_workspace_size(int nx, int nz);
-
 /* Adjoint of one RK stage of discrete_step (pyminiweather/solve/step.py:
  * 21-82) in one direction (0 = x, 1 = z), excluding the init term:
  * lam_out = scale * T'(state_forcing)^T lam_in + acc1 + acc2 + acc3, where
  * T is halo + reconstruction + flux + tendency (+ gravity in z) and scale is
  * the stage dt. Cotangent buffers have the state layout; only interiors are
  * read and written. acc* may be NULL; lam_out may alias an acc but not
- * lam_in. face_workspace: 8*(nz+1)*(nx+1) doubles. */
+ * lam_in or state_forcing. Periodic x. No reference counterpart (the
+ * reference only steps forward). */
 int pmw_adjoint_stage(int direction, const double* state_forcing, const double* lam_in,
                       double scale, const double* acc1, const double* acc2,
-                      const double* acc3, double* lam_out, double* face_workspace,
-                      const double* hy_dens_cell, const double* hy_dens_theta_cell,
-                      const double* hy_dens_int, const double* hy_dens_theta_int, int nx,
-                      int nz, double d, double hv_coeff, void* stream);
+                      const double* acc3, double* lam_out, const double* hy_dens_cell,
+                      const double* hy_dens_theta_cell, const double* hy_dens_int,
+                      const double* hy_dens_theta_int, int nx, int nz, double d,
+                      double hv_coeff, void* stream);
 
-/* Adjoint of one evolve step (pyminiweather/solve/step.py:85-143) taken
- * from `state` with the given direction order (reverse_direction as the
- * reference's toggle had it at that step): on entry lam is dL/d(state after
- * the step), on return dL/d(state before it) (interiors). Recomputes the
- * stage states from `state` (whose halos it rewrites) and reverses the two
- * directions in the opposite order, each stage with pmw_adjoint_stage.
- * Periodic x only. */
-int pmw_adjoint_step(double* state, double* lam, double* workspace,
-                     long long workspace_len, const double* hy_dens_cell,
-                     const double* hy_dens_theta_cell, const double* hy_dens_int,
-                     const double* hy_dens_theta_int, const double* hy_pressure_int,
-                     const double* src_w, int nx, int nz, double dx, double dz, double dt,
-                     double hv_coeff_x, double hv_coeff_z, int reverse_direction,
+/* FP64 entries of one tape slot: `members` states (4, nz+4, nx+4) back to
+ * back; 0 for invalid arguments. */
+long long pmw_tape_slot_len(int nx, int nz, int members);
+
+/* nsteps evolve steps (pyminiweather/solve/step.py:85-143, periodic x) of
+ * `members` identical grids that share the hydrostatic arrays and src_w,
+ * one fused launch per direction for all members. Slot 0 of `tape` holds
+ * the start states; step k writes slots 6k+1 .. 6k+6: stage 1 (only with
+ * store_stages) and stage 2 of its first direction, the state after it,
+ * stage 1 (with store_stages) and stage 2 of the second direction, and the
+ * state after the step. reverse_direction is the reference toggle at step
+ * 0 (nonzero = x first); it alternates per step. tape_len >= (6*nsteps+1)
+ * slots. tuning as pmw_evolve_ex (fused mode only; NULL = default). The
+ * interiors equal pmw_evolve_ex's bit for bit. No reference counterpart. */
+int pmw_tape_forward(double* tape, long long tape_len, int members,
+                     const double* hy_dens_cell, const double* hy_dens_theta_cell,
+                     const double* hy_dens_int, const double* hy_dens_theta_int,
+                     const double* hy_pressure_int, const double* src_w, int nx, int nz,
+                     double dx, double dz, double dt, double hv_coeff_x, double hv_coeff_z,
+                     int reverse_direction, int nsteps, int store_stages, const int* tuning,
                      void* stream);
+
+/* Reverse sweep over a tape written by pmw_tape_forward with store_stages:
+ * on entry lam (one slot) is dL/d(states after step nsteps-1), on return
+ * dL/d(start states) (interiors). Each step reverses its second direction,
+ * then its first; each direction is three pmw_adjoint_stage launches for
+ * all members. workspace: 2 slots. No reference counterpart. */
+int pmw_tape_adjoint(const double* tape, long long tape_len, int members, double* lam,
+                     double* workspace, long long workspace_len, const double* hy_dens_cell,
+                     const double* hy_dens_theta_cell, const double* hy_dens_int,
+                     const double* hy_dens_theta_int, int nx, int nz, double dx, double dz,
+                     double dt, double hv_coeff_x, double hv_coeff_z, int reverse_direction,
+                     int nsteps, void* stream);
 
 /* ---- reductions -------------------------------------------------------- */
 

@@ -56,11 +56,12 @@ SIGNATURES = {
     "pmw_ensemble_prepare": (_I, [_MP, _I, _IP, _P, _LL, _P]),
     "pmw_ensemble_evolve": (_I, [_MP, _I, _IP, _P, _IP, _I, _P,
                                  ctypes.POINTER(ctypes.c_double)]),
-    "pmw_adjoint_workspace_size": (_LL, [_I, _I]),
-    "pmw_adjoint_stage": (_I, [_I, _P, _P, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I,
-                               _D, _D, _P]),
-    "pmw_adjoint_step": (_I, [_P, _P, _P, _LL, _P, _P, _P, _P, _P, _P, _I, _I, _D, _D, _D,
-                              _D, _D, _I, _P]),
+    "pmw_adjoint_stage": (_I, [_I, _P, _P, _D, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _D, _D, _P]),
+    "pmw_tape_slot_len": (_LL, [_I, _I, _I]),
+    "pmw_tape_forward": (_I, [_P, _LL, _I, _P, _P, _P, _P, _P, _P, _I, _I, _D, _D, _D, _D, _D,
+                              _I, _I, _I, _P, _P]),
+    "pmw_tape_adjoint": (_I, [_P, _LL, _I, _P, _P, _LL, _P, _P, _P, _P, _I, _I, _D, _D, _D, _D,
+                              _D, _I, _I, _P]),
     "pmw_stats_workspace_size": (_LL, [_I, _I]),
     "pmw_compute_stats": (_I, [_P, _P, _P, _I, _I, _D, _D, _P, _LL, _P, _P]),
     "pmw_init": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P, _I, _P]),

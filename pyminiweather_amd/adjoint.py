@@ -1,27 +1,30 @@
 """Gradients through the time step (adjoint form).
 
-``AdjointSolver`` runs ``nsteps`` forward timesteps of one grid with the fused
-HIP kernels, keeping a checkpoint of the state every ``checkpoint_every``
-steps (default ceil(sqrt(nsteps))), and returns dL/d(initial state) for a
-cotangent of the final state by sweeping the segments backwards: each
-segment is recomputed forward from its checkpoint, then every step is
-reversed with the hand-written adjoint kernels (``pmw_adjoint_step``).
-Memory is O(sqrt(N)) states; recompute is one extra forward run.
+``AdjointSolver`` advances ``members`` identical grids (same size, profiles
+and source; one member by default) for ``nsteps`` timesteps with the fused
+HIP kernels, all members in one launch per direction, keeping a checkpoint
+of the states every ``checkpoint_every`` steps (default ceil(sqrt(nsteps))).
+The backward sweep takes the segments in reverse: each is recomputed from
+its checkpoint into a tape that holds every stage state of the segment
+(``pmw_tape_forward``), then swept backwards with the hand-written adjoint
+kernels (``pmw_tape_adjoint``, six launches per step for all members).
+Memory is O(sqrt(N)) states plus one segment's tape (6 states per step);
+the recompute is one extra forward run.
 
 ``EvolveFunction`` wraps the same as a ``torch.autograd.Function`` (initial
 interior state -> final interior state), so it composes with torch code.
 
-The assimilation problem of the adjoint form: the control is a per-cell
-theta' field mapped linearly into the initial (rho*theta)' by the per-row
-hydrostatic density, (rho*theta)'_{k,i} = hy_dens_cell[k] * theta'_{k,i}
-(the first-order map of ics/initial.py:73-77 with rho' = 0); the loss is
+The assimilation problem: the control is a per-cell theta' field mapped
+linearly into the initial (rho*theta)' by the per-row hydrostatic density,
+(rho*theta)'_{k,i} = hy_dens_cell[k] * theta'_{k,i} (the first-order map of
+ics/initial.py:73-77 with rho' = 0); the loss of a member is
 L = 1/2 * ||(rho*theta)'(T) - target||^2 over the interior.
 
 No reference counterpart: the reference only steps forward.
 """
-import ctypes
 import math
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -32,80 +35,81 @@ RHOT = 3
 
 
 class AdjointSolver:
-    def __init__(self, params, fields, nsteps, checkpoint_every=None, tuning=None):
+    def __init__(self, params, fields, nsteps, checkpoint_every=None, tuning=None, members=1):
         if params["ic_type"] == "injection":
             raise ValueError("the adjoint supports the periodic-x ICs only")
         self.p = dict(params)
         self.f = fields
+        self.members = int(members)
         self.nsteps = int(nsteps)
+        if self.nsteps < 1:
+            raise ValueError("nsteps must be >= 1")
         self.every = int(checkpoint_every or max(1, math.ceil(math.sqrt(self.nsteps))))
-        self.tuning = tuple(tuning) if tuning is not None else S.tuning_for(fields.nx,
-                                                                              fields.nz)
+        tun = tuple(tuning) if tuning is not None else S.tuning_for(fields.nx, fields.nz)
+        if tun[0] != 0:
+            tun = S.DEFAULT_TUNING
+        self.tuning = tun
+        lib = _lib.load()
+        slot = int(lib.pmw_tape_slot_len(fields.nx, fields.nz, self.members))
+        if slot <= 0:
+            raise ValueError("bad grid or member count")
+        self.bshape = (self.members,) + tuple(fields.shape)
         dev = fields.device
-        shape = fields.shape
-        self._buf = [torch.zeros(shape, dtype=torch.float64, device=dev) for _ in range(3)]
-        n = int(_lib.load().pmw_adjoint_workspace_size(fields.nx, fields.nz))
-        self._ws = torch.zeros(n, dtype=torch.float64, device=dev)
+        n = min(self.every, self.nsteps)
+        self._tape = torch.zeros((6 * n + 1,) + self.bshape, dtype=torch.float64, device=dev)
+        self._ws = torch.zeros((2,) + self.bshape, dtype=torch.float64, device=dev)
         self._hvx = S.hv_coeff(self.p["dx"], self.p["dt"])
         self._hvz = S.hv_coeff(self.p["dz"], self.p["dt"])
 
-    # -- forward ---------------------------------------------------------
-    def _steps(self, state, tmp, work, first_step, n):
-        """Advance `state` by n steps starting at step index first_step (the
-        direction order of step k is Z-first for even k, as the reference's
-        toggle gives in a fresh run)."""
-        f = self.f
-        flag = ctypes.c_int(first_step % 2)
-        tune = (ctypes.c_int * 3)(*self.tuning)
-        _lib.call("pmw_evolve_ex", ptr(state), ptr(tmp), ptr(work), ptr(f.hy_dens_cell),
-                  ptr(f.hy_dens_theta_cell), ptr(f.hy_dens_int), ptr(f.hy_dens_theta_int),
-                  ptr(f.hy_pressure_int), ptr(f.src_w) if f.src_w is not None else None, 0,
-                  None, f.nx, f.nz, float(self.p["dx"]), float(self.p["dz"]),
-                  float(self.p["dt"]), self._hvx, self._hvz, ctypes.byref(flag), int(n), tune,
-                  stream_of(f.device), None)
+    # -- helpers -----------------------------------------------------------
+    def _batch(self, x):
+        return x.reshape(self.bshape)
 
+    def _tape_run(self, start, first_step, n, store):
+        """Tape slot 0 <- start; n steps from step index first_step (the
+        direction order of step k is Z-first for even k, as the reference's
+        toggle gives in a fresh run). Returns the end-state slot (a view)."""
+        f = self.f
+        self._tape[0].copy_(start)
+        tune = (_lib.ctypes.c_int * 3)(*self.tuning)
+        _lib.call("pmw_tape_forward", ptr(self._tape), self._tape.numel(), self.members,
+                  ptr(f.hy_dens_cell), ptr(f.hy_dens_theta_cell), ptr(f.hy_dens_int),
+                  ptr(f.hy_dens_theta_int), ptr(f.hy_pressure_int),
+                  ptr(f.src_w) if f.src_w is not None else None, f.nx, f.nz,
+                  float(self.p["dx"]), float(self.p["dz"]), float(self.p["dt"]), self._hvx,
+                  self._hvz, first_step % 2, int(n), int(store), tune, stream_of(f.device))
+        return self._tape[6 * n]
+
+    # -- forward / backward ------------------------------------------------
     def forward(self, state0):
-        """Returns (final state, checkpoints). state0 is a state-shaped tensor
-        (only its interior matters)."""
-        s, t, w = self._buf
-        s.copy_(state0)
-        t.copy_(state0)
+        """Returns (final states, checkpoints); state0 has the state shape
+        (one member) or (members,) + state shape; only interiors matter."""
+        s = self._batch(state0)
         ckpts = []
         done = 0
         while done < self.nsteps:
             ckpts.append(s.clone())
             n = min(self.every, self.nsteps - done)
-            self._steps(s, t, w, done, n)
+            s = self._tape_run(s, done, n, store=False)
             done += n
-        return s.clone(), ckpts
+        return s.clone().reshape(state0.shape), ckpts
 
-    # -- backward --------------------------------------------------------
     def backward(self, lam_final, ckpts):
-        """dL/d(initial state) (interior; state layout) given dL/d(final
-        state) (interior of lam_final)."""
+        """dL/d(initial states) (interiors; state layout) given dL/d(final
+        states) (interior of lam_final)."""
         f = self.f
-        lam = lam_final.clone()
-        s, t, w = self._buf
-        nseg = len(ckpts)
-        for seg in reversed(range(nseg)):
+        lam = self._batch(lam_final).clone()
+        for seg in reversed(range(len(ckpts))):
             start = seg * self.every
             n = min(self.every, self.nsteps - start)
-            states = [ckpts[seg].clone()]
-            s.copy_(ckpts[seg])
-            t.copy_(ckpts[seg])
-            for k in range(1, n):
-                self._steps(s, t, w, start + k - 1, 1)
-                states.append(s.clone())
-            for k in reversed(range(n)):
-                step = start + k
-                _lib.call("pmw_adjoint_step", ptr(states[k]), ptr(lam), ptr(self._ws),
-                          self._ws.numel(), ptr(f.hy_dens_cell), ptr(f.hy_dens_theta_cell),
-                          ptr(f.hy_dens_int), ptr(f.hy_dens_theta_int),
-                          ptr(f.hy_pressure_int),
-                          ptr(f.src_w) if f.src_w is not None else None, f.nx, f.nz,
-                          float(self.p["dx"]), float(self.p["dz"]), float(self.p["dt"]),
-                          self._hvx, self._hvz, step % 2, stream_of(f.device))
-        return lam
+            self._tape_run(ckpts[seg], start, n, store=True)
+            _lib.call("pmw_tape_adjoint", ptr(self._tape), self._tape.numel(), self.members,
+                      ptr(lam), ptr(self._ws), self._ws.numel(), ptr(f.hy_dens_cell),
+                      ptr(f.hy_dens_theta_cell), ptr(f.hy_dens_int),
+                      ptr(f.hy_dens_theta_int), f.nx, f.nz, float(self.p["dx"]),
+                      float(self.p["dz"]), float(self.p["dt"]), self._hvx, self._hvz,
+                      start % 2, int(n), stream_of(f.device))
+        return lam.reshape(lam_final.shape)
 
     # -- the theta' assimilation problem ------------------------------------
     def row_weight(self):
@@ -113,45 +117,80 @@ class AdjointSolver:
         return self.f.hy_dens_cell[2:nz + 2]
 
     def state_from_theta(self, theta):
-        """Initial state with (rho*theta)' = hy_dens_cell[k] * theta' and the
-        other perturbations zero; theta is (nz, nx)."""
-        s0 = torch.zeros(self.f.shape, dtype=torch.float64, device=self.f.device)
-        s0[RHOT, 2:-2, 2:-2] = self.row_weight()[:, None] * theta
+        """Initial states with (rho*theta)' = hy_dens_cell[k] * theta' and
+        the other perturbations zero; theta is (nz, nx) or (members, nz, nx)."""
+        th = theta.reshape(self.members, self.f.nz, self.f.nx)
+        s0 = torch.zeros(self.bshape, dtype=torch.float64, device=self.f.device)
+        s0[:, RHOT, 2:-2, 2:-2] = self.row_weight()[:, None] * th
         return s0
 
+    def _losses(self, final, target):
+        resid = final[:, RHOT, 2:-2, 2:-2] - target.reshape(self.members, self.f.nz, self.f.nx)
+        return resid, 0.5 * torch.sum(resid * resid, dim=(1, 2))
+
     def loss_and_grad(self, theta, target):
-        """L(theta) = 1/2 ||(rho*theta)'(T) - target||^2 and dL/dtheta."""
+        """L(theta) = 1/2 ||(rho*theta)'(T) - target||^2 and dL/dtheta. For a
+        2-d theta: (float, (nz, nx) tensor); batched: ((members,) tensors)."""
         final, ckpts = self.forward(self.state_from_theta(theta))
-        resid = final[RHOT, 2:-2, 2:-2] - target
-        loss = 0.5 * float(torch.sum(resid * resid))
-        lam = torch.zeros_like(final)
-        lam[RHOT, 2:-2, 2:-2] = resid
+        resid, losses = self._losses(self._batch(final), target)
+        lam = torch.zeros(self.bshape, dtype=torch.float64, device=self.f.device)
+        lam[:, RHOT, 2:-2, 2:-2] = resid
         lam0 = self.backward(lam, ckpts)
-        return loss, lam0[RHOT, 2:-2, 2:-2] * self.row_weight()[:, None]
+        grad = (lam0[:, RHOT, 2:-2, 2:-2] * self.row_weight()[:, None]).reshape(theta.shape)
+        return (float(losses[0]), grad) if theta.dim() == 2 else (losses, grad)
 
     def loss(self, theta, target):
         final, _ = self.forward(self.state_from_theta(theta))
-        resid = final[RHOT, 2:-2, 2:-2] - target
-        return 0.5 * float(torch.sum(resid * resid))
+        _, losses = self._losses(self._batch(final), target)
+        return float(losses[0]) if theta.dim() == 2 else losses
+
+    def final_rhot(self, theta):
+        final, _ = self.forward(self.state_from_theta(theta))
+        return self._batch(final)[:, RHOT, 2:-2, 2:-2].reshape(
+            theta.shape if theta.dim() == 3 else (self.f.nz, self.f.nx))
+
+    def assimilate(self, theta0, target, iters=20):
+        """Steepest descent on every member's theta' at once: per iteration
+        one batched forward + checkpointed adjoint for the gradient g, then
+        the step alpha = <g, g> / ||J g||^2 (exact for the linearised
+        problem; J g from one more batched forward along g). theta0 and
+        target are (members, nz, nx). Returns (theta, losses per iteration
+        as a (iters + 1, members) array)."""
+        theta = theta0.clone()
+        history = []
+        m = self.members
+        for _ in range(iters):
+            losses, g = self.loss_and_grad(theta, target)
+            history.append(losses.cpu().numpy())
+            gmax = torch.amax(torch.abs(g).reshape(m, -1), dim=1).clamp_min(1e-300)
+            h = (1e-3 / gmax)[:, None, None]
+            base = self.final_rhot(theta)
+            jg = (self.final_rhot(theta + h * g) - base) / h
+            num = torch.sum((g * g).reshape(m, -1), dim=1)
+            den = torch.sum((jg * jg).reshape(m, -1), dim=1).clamp_min(1e-300)
+            theta = theta - (num / den)[:, None, None] * g
+        history.append(self.loss(theta, target).cpu().numpy())
+        return theta, np.stack(history)
 
 
 class EvolveFunction(torch.autograd.Function):
-    """Interior state (4, nz, nx) -> interior state after solver.nsteps steps,
-    with the HIP adjoint as its backward."""
+    """Interior state(s) (4, nz, nx) or (members, 4, nz, nx) -> the same after
+    solver.nsteps steps, with the HIP adjoint as its backward."""
 
     @staticmethod
     def forward(ctx, interior, solver):
-        s0 = torch.zeros(solver.f.shape, dtype=torch.float64, device=interior.device)
-        s0[:, 2:-2, 2:-2] = interior
+        s0 = torch.zeros(solver.bshape, dtype=torch.float64, device=interior.device)
+        s0[:, :, 2:-2, 2:-2] = interior.reshape(solver.members, 4, solver.f.nz, solver.f.nx)
         final, ckpts = solver.forward(s0)
         ctx.solver = solver
         ctx.ckpts = ckpts
-        return final[:, 2:-2, 2:-2].clone()
+        ctx.in_shape = interior.shape
+        return final[:, :, 2:-2, 2:-2].reshape(interior.shape).clone()
 
     @staticmethod
     def backward(ctx, grad):
         solver = ctx.solver
-        lam = torch.zeros(solver.f.shape, dtype=torch.float64, device=grad.device)
-        lam[:, 2:-2, 2:-2] = grad
+        lam = torch.zeros(solver.bshape, dtype=torch.float64, device=grad.device)
+        lam[:, :, 2:-2, 2:-2] = grad.reshape(solver.members, 4, solver.f.nz, solver.f.nx)
         lam0 = solver.backward(lam, ctx.ckpts)
-        return lam0[:, 2:-2, 2:-2].clone(), None
+        return lam0[:, :, 2:-2, 2:-2].reshape(ctx.in_shape).clone(), None

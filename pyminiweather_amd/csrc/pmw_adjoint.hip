@@ -5,22 +5,33 @@
 // where T_d is halo -> 4-tap reconstruction -> flux -> flux difference
 // (+ gravity in z). Its adjoint, for a cotangent lam of `out`:
 //   lam_init += lam,   lam_f += a_s * T_d'(f)^T lam.
-// T_d'(f)^T lam is computed in two passes:
-//  * face pass: for every face, lamF = (lamT(cell right) - lamT(cell left))
-//    / d, then the transposed 4x4 flux Jacobian at the face (from the
-//    forward face values of f) gives the cotangents of the reconstructed
-//    values and of the 3rd differences (8 numbers per face, to workspace);
-//  * cell pass: every interior cell gathers the 4-tap transposed
-//    reconstruction from the faces that read it, wrapping periodically in x
-//    and folding the wall halo rows of z onto the edge rows with the factors
-//    set_bc_z uses (1 for rho' and (rho*theta)', hy[halo]/hy[edge] for
-//    rho*u, 0 for rho*w); z adds the gravity term -g * lam_w to lam_rho'.
+// One launch computes a_s * T_d'(f)^T lam for a tile of cells:
+//  * every face of the tile (+1 face below, +2 above) forms
+//    lamF = a_s (lamT(cell after) - lamT(cell before)) / d and applies the
+//    transposed 4x4 flux Jacobian at the face (from the reconstructed
+//    forward values of f), giving the cotangents of its 4 reconstructed
+//    values and 4 third differences; these 8 numbers per face stay in LDS;
+//  * every cell then gathers the transposed 4-tap reconstruction from the 4
+//    faces that read it. In x the faces form a ring (face 0 and face nx read
+//    the same wrapped cells, so they are one face). In z the wall halo rows
+//    are functions of the edge rows (set_bc_z, ics/bcs.py:64-148): their
+//    cotangents fold back onto the edge rows with factors 1 for rho' and
+//    (rho*theta)', hy[halo]/hy[edge] for rho*u and 0 for rho*w. z adds the
+//    gravity term -g * lam_w to lam_rho'.
 // One RK3 direction: lamS2 = dt T'(S2)^T lam3; lamS1 = dt/2 T'(S1)^T lamS2;
 // lamS = dt/3 T'(S)^T lamS1 + lam3 + lamS2 + lamS1 (the init terms).
 // Cotangent buffers use the state layout (4, nz+4, nx+4); only the interior
 // is read or written. Derivatives are those of the real-number formulas the
 // reference evaluates (e.g. rho * (v/rho) is v).
+//
+// Tape: pmw_tape_forward advances `members` identical grids (one batched
+// launch per direction) and keeps, per step k, six state slots
+//   6k: S_k, 6k+1: S1 of the first direction, 6k+2: its S2, 6k+3: the state
+//   after the first direction, 6k+4 / 6k+5: S1 / S2 of the second direction,
+// with S_{k+1} in slot 6k+6; pmw_tape_adjoint sweeps the steps backwards
+// from it, six adjoint launches per step.
 #include "pmw_common.h"
+#include "pmw_fused.h"
 
 using namespace pmw;
 
@@ -28,6 +39,9 @@ namespace {
 
 constexpr double kW4[4] = {kW4a, kW4b, kW4b, kW4a};
 constexpr double kD3[4] = {-1.0, 3.0, -3.0, 1.0};
+constexpr int kAdjXThreads = 256;          // x: faces per block
+constexpr int kAdjXCells = kAdjXThreads - 3;  // x: cells per block
+constexpr int kAdjZRows = 8;               // z: cell rows per block (64 columns)
 
 __device__ __forceinline__ int wrapc(int c, int n) {
   int w = c % n;
@@ -78,44 +92,94 @@ __device__ __forceinline__ void adj_flux_z(const double val[4], double hyr, doub
   ld[3] = -hv * lf[3];
 }
 
-// ---- face passes: 8 cotangents per face into fw[8][faces] ---------------
-__global__ void k_adj_face_x(const double* __restrict__ f, const double* __restrict__ lam,
-                             double scale, const double* __restrict__ hyr,
-                             const double* __restrict__ hyt, double* __restrict__ fw,
-                             Grid g, double dx, double hv) {
-  const int nf = g.nx + 1;
-  const long long nfaces = (long long)g.nz * nf;
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nfaces) return;
-  const int k = (int)(idx / nf), face = (int)(idx % nf);
-  const int prow = k + kHs;
-  double val[4], d3[4];
+// Arguments of one adjoint stage; per-member buffers are mstride elements
+// apart (member = blockIdx.y).
+struct AdjArgs {
+  const double* f;     // forcing state of the stage
+  const double* lam;   // cotangent of the stage output
+  const double* acc1;  // nullable terms added to the result (same cell)
+  const double* acc2;
+  const double* acc3;
+  double* out;         // result; may alias an acc, never f or lam
+  const double* hy_cell;
+  const double* hyt_cell;
+  const double* hyr_int;
+  const double* hyt_int;
+  double scale, d, hv;
+  long long mstride;
+};
+
+__device__ __forceinline__ void adj_offset(AdjArgs& a) {
+  const long long off = (long long)blockIdx.y * a.mstride;
+  a.f += off;
+  a.lam += off;
+  a.out += off;
+  if (a.acc1) a.acc1 += off;
+  if (a.acc2) a.acc2 += off;
+  if (a.acc3) a.acc3 += off;
+}
+
+__device__ __forceinline__ void finish_cell(const AdjArgs& a, const Grid& g, int prow, int pc,
+                                            const double l[4]) {
 #pragma unroll
   for (int v = 0; v < kNv; ++v) {
-    double s[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) s[j] = f[g.at(v, prow, wrapc(face - 2 + j, g.nx) + kHs)];
-    recon4(s[0], s[1], s[2], s[3], val[v], d3[v]);
-  }
-  // T(i) = (F(i) - F(i+1)) / dx: face `face` is F(i) of cell face and F(i+1)
-  // of cell face-1
-  double lf[4];
-#pragma unroll
-  for (int v = 0; v < kNv; ++v) {
-    const double right = face < g.nx ? lam[g.at(v, prow, face + kHs)] : 0.0;
-    const double left = face > 0 ? lam[g.at(v, prow, face - 1 + kHs)] : 0.0;
-    lf[v] = scale * (right - left) / dx;
-  }
-  double lv[4], ld[4];
-  adj_flux_x(val, hyr[prow], hyt[prow], hv, lf, lv, ld);
-#pragma unroll
-  for (int v = 0; v < kNv; ++v) {
-    fw[v * nfaces + idx] = lv[v];
-    fw[(4 + v) * nfaces + idx] = ld[v];
+    const long long o = g.at(v, prow, pc);
+    double r = l[v];
+    if (a.acc1) r += a.acc1[o];
+    if (a.acc2) r += a.acc2[o];
+    if (a.acc3) r += a.acc3[o];
+    a.out[o] = r;
   }
 }
 
-// wall halo value of padded row ph for variable v given the edge row value
+// ---- x: block = one row, faces c0-1 .. c0+kAdjXThreads-2 (ring, wrapped),
+// cells c0 .. c0+kAdjXCells-1
+__global__ __launch_bounds__(kAdjXThreads) void k_adj_x(AdjArgs a, Grid g, int ntiles) {
+  __shared__ double fw[8][kAdjXThreads];
+  adj_offset(a);
+  const int nx = g.nx, t = threadIdx.x;
+  const int k = blockIdx.x / ntiles, c0 = (blockIdx.x % ntiles) * kAdjXCells;
+  const int prow = k + kHs;
+  {
+    const int face = c0 - 1 + t;  // reads cells face-2 .. face+1
+    double val[4], d3[4], lf[4], lv[4], ld[4];
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) {
+      double s[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) s[j] = a.f[g.at(v, prow, wrapc(face - 2 + j, nx) + kHs)];
+      recon4(s[0], s[1], s[2], s[3], val[v], d3[v]);
+    }
+    // T(i) = (F(i) - F(i+1)) / dx: the face is F(i) of cell `face` and
+    // F(i+1) of cell face-1
+    const int cr = wrapc(face, nx) + kHs, cl = wrapc(face - 1, nx) + kHs;
+#pragma unroll
+    for (int v = 0; v < kNv; ++v)
+      lf[v] = a.scale * (a.lam[g.at(v, prow, cr)] - a.lam[g.at(v, prow, cl)]) / a.d;
+    adj_flux_x(val, a.hy_cell[prow], a.hyt_cell[prow], a.hv, lf, lv, ld);
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) {
+      fw[v][t] = lv[v];
+      fw[4 + v][t] = ld[v];
+    }
+  }
+  __syncthreads();
+  const int c = c0 + t;
+  if (t < kAdjXCells && c < nx) {
+    // tap j of face c+2-j reads cell c; that face is slot t+3-j
+    double l[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int s = t + 3 - j;
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) l[v] += kW4[j] * fw[v][s] + kD3[j] * fw[4 + v][s];
+    }
+    finish_cell(a, g, prow, c + kHs, l);
+  }
+}
+
+// ---- z: block = 64 columns x kAdjZRows cell rows r0.. ; face slots
+// s = 0 .. R+2 hold faces r0-1+s (those in 0..nz)
 __device__ __forceinline__ double zrow_value(const double* f, const Grid& g,
                                              const double* hy, int v, int r, int pc) {
   if (r >= 0 && r < g.nz) return f[g.at(v, r + kHs, pc)];
@@ -123,242 +187,228 @@ __device__ __forceinline__ double zrow_value(const double* f, const Grid& g,
   return wall_halo(v, f[g.at(v, pe, pc)], hy[pe], hy[r + kHs]);
 }
 
-__global__ void k_adj_face_z(const double* __restrict__ f, const double* __restrict__ lam,
-                             double scale, const double* __restrict__ hy_cell,
-                             const double* __restrict__ hyr, const double* __restrict__ hyt,
-                             double* __restrict__ fw, Grid g, double dz, double hv) {
-  const int nx = g.nx;
-  const long long nfaces = (long long)(g.nz + 1) * nx;
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nfaces) return;
-  const int face = (int)(idx / nx), i = (int)(idx % nx);
-  const int pc = i + kHs;
-  double val[4], d3[4];
-#pragma unroll
-  for (int v = 0; v < kNv; ++v) {
-    double s[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) s[j] = zrow_value(f, g, hy_cell, v, face - 2 + j, pc);
-    recon4(s[0], s[1], s[2], s[3], val[v], d3[v]);
-  }
-  double lf[4];
-#pragma unroll
-  for (int v = 0; v < kNv; ++v) {
-    const double above = face < g.nz ? lam[g.at(v, face + kHs, pc)] : 0.0;
-    const double below = face > 0 ? lam[g.at(v, face - 1 + kHs, pc)] : 0.0;
-    lf[v] = scale * (above - below) / dz;
-  }
-  double lv[4], ld[4];
-  adj_flux_z(val, hyr[face], hyt[face], hv, face == 0 || face == g.nz, lf, lv, ld);
-#pragma unroll
-  for (int v = 0; v < kNv; ++v) {
-    fw[v * nfaces + idx] = lv[v];
-    fw[(4 + v) * nfaces + idx] = ld[v];
-  }
-}
-
-// ---- cell passes ---------------------------------------------------------
-// lam_out = T'^T part + acc1 + acc2 + acc3 (accs nullable; lam_out may alias
-// any acc, which is read only at the same cell).
-__global__ void k_adj_cell_x(const double* __restrict__ fw, const double* acc1,
-                             const double* acc2, const double* acc3, double* lam_out,
-                             Grid g) {
-  const int nx = g.nx, nf = nx + 1;
-  const long long nfaces = (long long)g.nz * nf;
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)g.nz * nx) return;
-  const int k = (int)(idx / nx), c = (int)(idx % nx);
-  double l[4] = {0.0, 0.0, 0.0, 0.0};
-  // every copy of cell c among padded columns -2..nx+1 (periodic halo)
-  for (int m = -1; m <= 1; ++m) {
-    const int cp = c + m * nx;
-    if (cp < -2 || cp > nx + 1) continue;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int face = cp + 2 - j;
-      if (face < 0 || face > nx) continue;
-      const long long fi = (long long)k * nf + face;
-#pragma unroll
-      for (int v = 0; v < kNv; ++v)
-        l[v] += kW4[j] * fw[v * nfaces + fi] + kD3[j] * fw[(4 + v) * nfaces + fi];
-    }
-  }
-#pragma unroll
-  for (int v = 0; v < kNv; ++v) {
-    const long long o = g.at(v, k + kHs, c + kHs);
-    double r = l[v];
-    if (acc1) r += acc1[o];
-    if (acc2) r += acc2[o];
-    if (acc3) r += acc3[o];
-    lam_out[o] = r;
-  }
-}
-
-__device__ __forceinline__ void gather_z_row(const double* fw, long long nfaces, int nx,
-                                             int nz, int r, int i, double l[4]) {
-  // cotangent of forcing row r (-2 <= r <= nz+1) of column i
+template <int R>
+__device__ __forceinline__ void gather_z_row(const double (*fw)[R + 3][64], int nz, int r,
+                                             int r0, int lane, double l[4]) {
+  // cotangent of forcing row r (-2 <= r <= nz+1) of this lane's column
 #pragma unroll
   for (int v = 0; v < kNv; ++v) l[v] = 0.0;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int face = r + 2 - j;
     if (face < 0 || face > nz) continue;
-    const long long fi = (long long)face * nx + i;
+    const int s = face - r0 + 1;
 #pragma unroll
-    for (int v = 0; v < kNv; ++v)
-      l[v] += kW4[j] * fw[v * nfaces + fi] + kD3[j] * fw[(4 + v) * nfaces + fi];
+    for (int v = 0; v < kNv; ++v) l[v] += kW4[j] * fw[v][s][lane] + kD3[j] * fw[4 + v][s][lane];
   }
 }
 
-__global__ void k_adj_cell_z(const double* __restrict__ fw, const double* __restrict__ lam,
-                             double scale, const double* __restrict__ hy, const double* acc1,
-                             const double* acc2, const double* acc3, double* lam_out,
-                             Grid g) {
+template <int R>
+__global__ __launch_bounds__(256) void k_adj_z(AdjArgs a, Grid g, int ncol) {
+  constexpr int NF = R + 3;
+  __shared__ double fw[8][NF][64];
+  adj_offset(a);
   const int nx = g.nx, nz = g.nz;
-  const long long nfaces = (long long)(nz + 1) * nx;
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)nz * nx) return;
-  const int k = (int)(idx / nx), i = (int)(idx % nx);
-  double l[4], h[4];
-  gather_z_row(fw, nfaces, nx, nz, k, i, l);
-  // wall halo rows are functions of the edge row (set_bc_z): fold them back
-  if (k == 0 || k == nz - 1) {
-    const int pe = k + kHs;
-    const int r0 = k == 0 ? -2 : nz;
-    for (int r = r0; r < r0 + 2; ++r) {
-      gather_z_row(fw, nfaces, nx, nz, r, i, h);
-      const int ph = r + kHs;
-      l[0] += h[0];
-      l[1] += h[1] * hy[ph] / hy[pe];
-      l[3] += h[3];
+  const int lane = threadIdx.x & 63, wy = threadIdx.x >> 6;
+  const int i = (blockIdx.x % ncol) * 64 + lane, r0 = (blockIdx.x / ncol) * R;
+  const int pc = i + kHs;
+  if (i < nx) {
+    for (int s = wy; s < NF; s += 4) {
+      const int face = r0 - 1 + s;
+      if (face < 0 || face > nz) continue;
+      double val[4], d3[4], lf[4], lv[4], ld[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) {
+        double q[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[j] = zrow_value(a.f, g, a.hy_cell, v, face - 2 + j, pc);
+        recon4(q[0], q[1], q[2], q[3], val[v], d3[v]);
+      }
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) {
+        const double above = face < nz ? a.lam[g.at(v, face + kHs, pc)] : 0.0;
+        const double below = face > 0 ? a.lam[g.at(v, face - 1 + kHs, pc)] : 0.0;
+        lf[v] = a.scale * (above - below) / a.d;
+      }
+      adj_flux_z(val, a.hyr_int[face], a.hyt_int[face], a.hv, face == 0 || face == nz, lf, lv,
+                 ld);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) {
+        fw[v][s][lane] = lv[v];
+        fw[4 + v][s][lane] = ld[v];
+      }
     }
   }
-  // gravity: T_w -= rho'_forcing * g
-  l[0] += -kGrav * (scale * lam[g.at(2, k + kHs, i + kHs)]);
-#pragma unroll
-  for (int v = 0; v < kNv; ++v) {
-    const long long o = g.at(v, k + kHs, i + kHs);
-    double r = l[v];
-    if (acc1) r += acc1[o];
-    if (acc2) r += acc2[o];
-    if (acc3) r += acc3[o];
-    lam_out[o] = r;
+  __syncthreads();
+  if (i >= nx) return;
+  for (int rr = wy; rr < R; rr += 4) {
+    const int k = r0 + rr;
+    if (k >= nz) break;
+    double l[4], h[4];
+    gather_z_row<R>(fw, nz, k, r0, lane, l);
+    if (k == 0 || k == nz - 1) {
+      // the wall halo rows of this edge row
+      const int pe = k + kHs;
+      const int rh = k == 0 ? -2 : nz;
+      for (int r = rh; r < rh + 2; ++r) {
+        gather_z_row<R>(fw, nz, r, r0, lane, h);
+        l[0] += h[0];
+        l[1] += h[1] * a.hy_cell[r + kHs] / a.hy_cell[pe];
+        l[3] += h[3];
+      }
+    }
+    // gravity: T_w -= rho'_forcing * g
+    l[0] += -kGrav * (a.scale * a.lam[g.at(2, k + kHs, pc)]);
+    finish_cell(a, g, k + kHs, pc, l);
   }
 }
 
-inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
-
 int adj_stage(int dir, const double* f, const double* lam, double scale, const double* acc1,
-              const double* acc2, const double* acc3, double* lam_out, double* fw,
+              const double* acc2, const double* acc3, double* lam_out,
               const double* hy_cell, const double* hyt_cell, const double* hyr_int,
-              const double* hyt_int, Grid g, double d, double hv, hipStream_t st) {
+              const double* hyt_int, Grid g, double d, double hv, int members,
+              long long mstride, hipStream_t st) {
+  AdjArgs a{f, lam, acc1, acc2, acc3, lam_out, hy_cell, hyt_cell, hyr_int, hyt_int,
+            scale, d, hv, mstride};
   if (dir == 0) {
-    const long long nfaces = (long long)g.nz * (g.nx + 1);
-    k_adj_face_x<<<nblk(nfaces), 256, 0, st>>>(f, lam, scale, hy_cell, hyt_cell, fw, g, d, hv);
-    PMW_CHECK_LAUNCH();
-    k_adj_cell_x<<<nblk((long long)g.nz * g.nx), 256, 0, st>>>(fw, acc1, acc2, acc3,
-                                                              lam_out, g);
+    const int ntiles = (g.nx + kAdjXCells - 1) / kAdjXCells;
+    k_adj_x<<<dim3((unsigned)(ntiles * g.nz), members), kAdjXThreads, 0, st>>>(a, g, ntiles);
   } else {
-    const long long nfaces = (long long)(g.nz + 1) * g.nx;
-    k_adj_face_z<<<nblk(nfaces), 256, 0, st>>>(f, lam, scale, hy_cell, hyr_int, hyt_int, fw,
-                                             g, d, hv);
-    PMW_CHECK_LAUNCH();
-    k_adj_cell_z<<<nblk((long long)g.nz * g.nx), 256, 0, st>>>(fw, lam, scale, hy_cell, acc1,
-                                                              acc2, acc3, lam_out, g);
+    const int ncol = (g.nx + 63) / 64;
+    const int nrt = (g.nz + kAdjZRows - 1) / kAdjZRows;
+    k_adj_z<kAdjZRows><<<dim3((unsigned)(ncol * nrt), members), 256, 0, st>>>(a, g, ncol);
   }
   PMW_CHECK_LAUNCH();
   return PMW_OK;
 }
 
-bool grid_ok(int nx, int nz) { return nx >= 2 && nz >= 2; }
+bool grid_ok(int nx, int nz) {
+  // blocks per launch stay below 2^31 and rows index as int
+  return nx >= 2 && nz >= 2 && (long long)(nx + 4) * (nz + 4) * 4 < (1LL << 40);
+}
+
+long long state_len(int nx, int nz) { return 4LL * (nz + 4) * (nx + 4); }
 
 }  // namespace
 
-// Forward stage kernels of pmw_step.hip (one RK stage, fused).
-extern "C" int pmw_rk_stage(int direction, const double* state_init, double* state_forcing,
-                            double* state_out, const double* hy_dens_cell,
-                            const double* hy_dens_theta_cell, const double* hy_dens_int,
-                            const double* hy_dens_theta_int, const double* hy_pressure_int,
-                            const double* src_w, int bc, const int* inj_rows, int nx, int nz,
-                            double d, double dt_stage, double hv_coeff, void* stream);
-
 extern "C" {
-
-long long pmw_adjoint_workspace_size(int nx, int nz) {
-  if (!grid_ok(nx, nz)) return 0;
-  const long long state = 4LL * (nz + 4) * (nx + 4);
-  const long long faces = 8LL * (nz + 1) * (nx + 1);
-  return 5 * state + faces;
-}
 
 int pmw_adjoint_stage(int direction, const double* state_forcing, const double* lam_in,
                       double scale, const double* acc1, const double* acc2,
-                      const double* acc3, double* lam_out, double* face_workspace,
-                      const double* hy_dens_cell, const double* hy_dens_theta_cell,
-                      const double* hy_dens_int, const double* hy_dens_theta_int, int nx,
-                      int nz, double d, double hv_coeff, void* stream) {
-  if (!grid_ok(nx, nz) || !state_forcing || !lam_in || !lam_out || !face_workspace ||
-      !hy_dens_cell || (direction == 0 && !hy_dens_theta_cell) ||
+                      const double* acc3, double* lam_out, const double* hy_dens_cell,
+                      const double* hy_dens_theta_cell, const double* hy_dens_int,
+                      const double* hy_dens_theta_int, int nx, int nz, double d,
+                      double hv_coeff, void* stream) {
+  if (!grid_ok(nx, nz) || !state_forcing || !lam_in || !lam_out || !hy_dens_cell ||
+      (direction == 0 && !hy_dens_theta_cell) ||
       (direction == 1 && (!hy_dens_int || !hy_dens_theta_int)))
     return PMW_EINVAL;
   if (direction != 0 && direction != 1) return PMW_EINVAL;
-  if (lam_out == lam_in) return PMW_EALIAS;
+  if (lam_out == lam_in || lam_out == state_forcing) return PMW_EALIAS;
   return adj_stage(direction, state_forcing, lam_in, scale, acc1, acc2, acc3, lam_out,
-                   face_workspace, hy_dens_cell, hy_dens_theta_cell, hy_dens_int,
-                   hy_dens_theta_int, Grid(nx, nz), d, hv_coeff, (hipStream_t)stream);
+                   hy_dens_cell, hy_dens_theta_cell, hy_dens_int, hy_dens_theta_int,
+                   Grid(nx, nz), d, hv_coeff, 1, 0, (hipStream_t)stream);
 }
 
-int pmw_adjoint_step(double* state, double* lam, double* workspace,
-                     long long workspace_len, const double* hy_dens_cell,
-                     const double* hy_dens_theta_cell, const double* hy_dens_int,
-                     const double* hy_dens_theta_int, const double* hy_pressure_int,
-                     const double* src_w, int nx, int nz, double dx, double dz, double dt,
-                     double hv_coeff_x, double hv_coeff_z, int reverse_direction,
+long long pmw_tape_slot_len(int nx, int nz, int members) {
+  if (!grid_ok(nx, nz) || members < 1 || members > 65535) return 0;
+  return state_len(nx, nz) * members;
+}
+
+int pmw_tape_forward(double* tape, long long tape_len, int members,
+                     const double* hy_dens_cell, const double* hy_dens_theta_cell,
+                     const double* hy_dens_int, const double* hy_dens_theta_int,
+                     const double* hy_pressure_int, const double* src_w, int nx, int nz,
+                     double dx, double dz, double dt, double hv_coeff_x, double hv_coeff_z,
+                     int reverse_direction, int nsteps, int store_stages, const int* tuning,
                      void* stream) {
-  if (!grid_ok(nx, nz) || !state || !lam || !workspace || !hy_dens_cell ||
-      !hy_dens_theta_cell || !hy_dens_int || !hy_dens_theta_int || !hy_pressure_int)
+  const long long slot = pmw_tape_slot_len(nx, nz, members);
+  if (slot == 0 || !tape || !hy_dens_cell || !hy_dens_theta_cell || !hy_dens_int ||
+      !hy_dens_theta_int || !hy_pressure_int || nsteps < 0 || nsteps > 100000)
     return PMW_EINVAL;
-  if (workspace_len < pmw_adjoint_workspace_size(nx, nz)) return PMW_ESPACE;
-  const long long ns = 4LL * (nz + 4) * (nx + 4);
-  double* mid = workspace;        // state after the first direction
-  double* s1 = workspace + ns;    // stage-1 state of the direction being reversed
-  double* s2 = workspace + 2 * ns;
-  double* l1 = workspace + 3 * ns;
-  double* l2 = workspace + 4 * ns;
-  double* fw = workspace + 5 * ns;
-  double* s0 = state;  // its halos are rewritten (set_bc of the forward stages)
+  int x_nt = 256, z_seg = 16;
+  if (tuning) {
+    if (tuning[0] != 0 || !fused_x_tile_ok(tuning[1]) || !fused_z_seg_ok(tuning[2]))
+      return PMW_EINVAL;
+    x_nt = tuning[1];
+    z_seg = tuning[2];
+  }
+  if (tape_len < (6LL * nsteps + 1) * slot) return PMW_ESPACE;
   Grid g(nx, nz);
   hipStream_t st = (hipStream_t)stream;
+  DirArgs a;
+  a.hy_cell = hy_dens_cell;
+  a.hyt_cell = hy_dens_theta_cell;
+  a.hyr_int = hy_dens_int;
+  a.hyt_int = hy_dens_theta_int;
+  a.hyp_int = hy_pressure_int;
+  a.src_w = src_w;
+  a.dt1 = dt / 3;
+  a.dt2 = dt / 2;
+  a.dt3 = dt / 1;
+  a.mstride = state_len(nx, nz);
+  for (int k = 0; k < nsteps; ++k) {
+    const int rev = (reverse_direction != 0) ^ (k & 1);
+    const int first = rev ? 0 : 1;  // 0 = x, 1 = z
+    for (int pass = 0; pass < 2; ++pass) {
+      const int dir = pass == 0 ? first : 1 - first;
+      double* base = tape + (6LL * k + 3 * pass) * slot;
+      a.in = base;
+      a.st = base;  // halo refresh of the start state (never read by this launch)
+      a.s1 = store_stages ? base + slot : nullptr;
+      a.tmp = base + 2 * slot;
+      a.out = base + 3 * slot;
+      a.d = dir == 0 ? dx : dz;
+      a.hv = dir == 0 ? hv_coeff_x : hv_coeff_z;
+      const int rc = dir == 0 ? launch_dir_x(a, g, x_nt, st, members)
+                              : launch_dir_z(a, g, z_seg, st, members);
+      if (rc) return rc;
+    }
+  }
+  return PMW_OK;
+}
+
+int pmw_tape_adjoint(const double* tape, long long tape_len, int members, double* lam,
+                     double* workspace, long long workspace_len, const double* hy_dens_cell,
+                     const double* hy_dens_theta_cell, const double* hy_dens_int,
+                     const double* hy_dens_theta_int, int nx, int nz, double dx, double dz,
+                     double dt, double hv_coeff_x, double hv_coeff_z, int reverse_direction,
+                     int nsteps, void* stream) {
+  const long long slot = pmw_tape_slot_len(nx, nz, members);
+  if (slot == 0 || !tape || !lam || !workspace || !hy_dens_cell || !hy_dens_theta_cell ||
+      !hy_dens_int || !hy_dens_theta_int || nsteps < 0 || nsteps > 100000)
+    return PMW_EINVAL;
+  if (tape_len < (6LL * nsteps + 1) * slot || workspace_len < 2 * slot) return PMW_ESPACE;
+  const double* tape_end = tape + (6LL * nsteps + 1) * slot;
+  if ((lam >= tape && lam < tape_end) || (workspace >= tape && workspace < tape_end) ||
+      (lam >= workspace && lam < workspace + 2 * slot))
+    return PMW_EALIAS;
+  Grid g(nx, nz);
+  hipStream_t st = (hipStream_t)stream;
+  double* l1 = workspace;
+  double* l2 = workspace + slot;
   const double dts[3] = {dt / 3, dt / 2, dt / 1};
-  const int first = reverse_direction ? 0 : 1;  // 0 = x, 1 = z
-  const int dirs[2] = {first, 1 - first};
-  auto fwd = [&](int dir, double* init, double* forc, double* out, double dts_) {
-    return pmw_rk_stage(dir, init, forc, out, hy_dens_cell, hy_dens_theta_cell, hy_dens_int,
-                        hy_dens_theta_int, hy_pressure_int, src_w, PMW_BC_PERIODIC, nullptr,
-                        nx, nz, dir == 0 ? dx : dz, dts_, dir == 0 ? hv_coeff_x : hv_coeff_z,
-                        st);
-  };
-  // the state after the first direction: S -> s1 -> s2 -> mid
-  int rc;
-  if ((rc = fwd(dirs[0], s0, s0, s1, dts[0]))) return rc;
-  if ((rc = fwd(dirs[0], s0, s1, s2, dts[1]))) return rc;
-  if ((rc = fwd(dirs[0], s0, s2, mid, dts[2]))) return rc;
-  for (int pass = 1; pass >= 0; --pass) {
-    const int dir = dirs[pass];
-    double* start = pass == 1 ? mid : s0;
-    // stage states of this direction
-    if ((rc = fwd(dir, start, start, s1, dts[0]))) return rc;
-    if ((rc = fwd(dir, start, s1, s2, dts[1]))) return rc;
-    const double d = dir == 0 ? dx : dz;
-    const double hv = dir == 0 ? hv_coeff_x : hv_coeff_z;
-    auto adj = [&](const double* f, const double* lin, double sc, const double* a1,
-                   const double* a2, const double* a3, double* lout) {
-      return adj_stage(dir, f, lin, sc, a1, a2, a3, lout, fw, hy_dens_cell,
-                       hy_dens_theta_cell, hy_dens_int, hy_dens_theta_int, g, d, hv, st);
-    };
-    if ((rc = adj(s2, lam, dts[2], nullptr, nullptr, nullptr, l2))) return rc;
-    if ((rc = adj(s1, l2, dts[1], nullptr, nullptr, nullptr, l1))) return rc;
-    if ((rc = adj(start, l1, dts[0], lam, l2, l1, lam))) return rc;
+  const long long ms = state_len(nx, nz);
+  for (int k = nsteps - 1; k >= 0; --k) {
+    const int rev = (reverse_direction != 0) ^ (k & 1);
+    const int first = rev ? 0 : 1;
+    for (int pass = 1; pass >= 0; --pass) {
+      const int dir = pass == 0 ? first : 1 - first;
+      const double* s0 = tape + (6LL * k + 3 * pass) * slot;
+      const double* s1 = s0 + slot;
+      const double* s2 = s0 + 2 * slot;
+      const double d = dir == 0 ? dx : dz;
+      const double hv = dir == 0 ? hv_coeff_x : hv_coeff_z;
+      auto adj = [&](const double* f, const double* lin, double sc, const double* a1,
+                     const double* a2, const double* a3, double* lout) {
+        return adj_stage(dir, f, lin, sc, a1, a2, a3, lout, hy_dens_cell, hy_dens_theta_cell,
+                         hy_dens_int, hy_dens_theta_int, g, d, hv, members, ms, st);
+      };
+      int rc;
+      if ((rc = adj(s2, lam, dts[2], nullptr, nullptr, nullptr, l2))) return rc;
+      if ((rc = adj(s1, l2, dts[1], nullptr, nullptr, nullptr, l1))) return rc;
+      if ((rc = adj(s0, l1, dts[0], lam, l2, l1, lam))) return rc;
+    }
   }
   return PMW_OK;
 }

@@ -20,11 +20,16 @@ struct DirArgs {
   double d;                // dx or dz
   double dt1, dt2, dt3;    // dt/3, dt/2, dt
   double hv;               // -hv_beta * d / (16 * dt_full)
+  double* s1 = nullptr;    // optional: receives the stage-1 interior
+  long long mstride = 0;   // batched launches: elements between members of
+                           // in / st / out / tmp / s1 (member = blockIdx.y)
 };
 
 bool fused_x_tile_ok(int nt);
 bool fused_z_seg_ok(int seg);
-int launch_dir_x(const DirArgs& a, Grid g, int nt, hipStream_t st);
-int launch_dir_z(const DirArgs& a, Grid g, int seg, hipStream_t st);
+// `members` identical grids (same hydrostatic profiles and source) laid out
+// a.mstride elements apart are advanced by one launch.
+int launch_dir_x(const DirArgs& a, Grid g, int nt, hipStream_t st, int members = 1);
+int launch_dir_z(const DirArgs& a, Grid g, int seg, hipStream_t st, int members = 1);
 
 }  // namespace pmw

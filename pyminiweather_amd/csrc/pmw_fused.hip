@@ -104,6 +104,10 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
     if (a.src_w) td[2] = td[2] + src;
 #pragma unroll
     for (int v = 0; v < kNv; ++v) s1[v][t] = s0[v][t] + a.dt1 * td[v];
+    if (a.s1 && cell >= i0 && cell < i0 + C && cell < nx) {
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) a.s1[g.at(v, prow, cell + kHs)] = s1[v][t];
+    }
   }
   __syncthreads();
   // ---- stage 2: faces 4..NT-4, cells 4..NT-5, s1 -> s1 (in place)
@@ -165,6 +169,16 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
   }
 }
 
+// Member blockIdx.y of a batched launch: offset every per-member buffer.
+__device__ __forceinline__ void member_offset(DirArgs& a) {
+  const long long off = (long long)blockIdx.y * a.mstride;
+  a.in += off;
+  a.st += off;
+  a.out += off;
+  a.tmp += off;
+  if (a.s1) a.s1 += off;
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) void k_dir_x(DirArgs a, Grid g, int ntiles, int nreal,
                                               int per_xcd) {
@@ -172,6 +186,7 @@ __global__ __launch_bounds__(NT) void k_dir_x(DirArgs a, Grid g, int ntiles, int
   __shared__ XTileLds<NT> sm;
   const int logical = xcd_logical(blockIdx.x, per_xcd);
   if (logical >= nreal) return;  // whole block, before any barrier
+  member_offset(a);
   dir_x_tile<NT>(a, g, logical / ntiles, (logical % ntiles) * C, sm);
 }
 
@@ -314,6 +329,7 @@ __device__ __forceinline__ void z_block_step(double w[4][4], double F[4], const 
 #pragma unroll
         for (int v = 0; v < kNv; ++v) ring[stage][c & 1][v][lane] = res[v];
       }
+      if (stage == 0 && a.s1 && c >= k0 && c < k1) store_row(a.s1, g, c + kHs, pc, res);
       if (stage == 1) {
         if (c >= k0 && c < k1) store_row(a.tmp, g, c + kHs, pc, res);
         if (c == 0 && k0 == 0) {  // set_bc_z of stage 3 into tmp
@@ -425,6 +441,7 @@ __global__ __launch_bounds__(192) void k_dir_z(DirArgs a, Grid g, int ncolblk, i
   __shared__ double ring[2][2][kNv][64];
   const int logical = xcd_logical(blockIdx.x, per_xcd);
   if (logical >= nreal) return;  // whole block, before any barrier
+  member_offset(a);
   dir_z_block<L>(a, g, logical % ncolblk, logical / ncolblk, ring);
 }
 
@@ -523,23 +540,23 @@ __global__ __launch_bounds__(192) void k_ens_z(const MemberDev* __restrict__ m, 
 }
 
 template <int NT>
-int launch_x(const DirArgs& a, Grid g, hipStream_t st) {
+int launch_x(const DirArgs& a, Grid g, hipStream_t st, int members) {
   constexpr int C = NT - 12;
   const int ntiles = (g.nx + C - 1) / C;
   const long long nreal = (long long)ntiles * g.nz;
   const int per = (int)((nreal + kNumXcd - 1) / kNumXcd);
-  k_dir_x<NT><<<per * kNumXcd, NT, 0, st>>>(a, g, ntiles, (int)nreal, per);
+  k_dir_x<NT><<<dim3(per * kNumXcd, members), NT, 0, st>>>(a, g, ntiles, (int)nreal, per);
   PMW_CHECK_LAUNCH();
   return PMW_OK;
 }
 
 template <int L>
-int launch_z(const DirArgs& a, Grid g, hipStream_t st) {
+int launch_z(const DirArgs& a, Grid g, hipStream_t st, int members) {
   const int ncolblk = (g.nx + 4 + 63) / 64;
   const int nseg = (g.nz + L - 1) / L;
   const long long nreal = (long long)ncolblk * nseg;
   const int per = (int)((nreal + kNumXcd - 1) / kNumXcd);
-  k_dir_z<L><<<per * kNumXcd, 192, 0, st>>>(a, g, ncolblk, (int)nreal, per);
+  k_dir_z<L><<<dim3(per * kNumXcd, members), 192, 0, st>>>(a, g, ncolblk, (int)nreal, per);
   PMW_CHECK_LAUNCH();
   return PMW_OK;
 }
@@ -553,23 +570,25 @@ bool fused_z_seg_ok(int seg) {
   return seg == 8 || seg == 16 || seg == 24 || seg == 32 || seg == 48 || seg == 64;
 }
 
-int launch_dir_x(const DirArgs& a, Grid g, int nt, hipStream_t st) {
+int launch_dir_x(const DirArgs& a, Grid g, int nt, hipStream_t st, int members) {
+  if (members < 1 || members > 65535 || (members > 1 && a.mstride <= 0)) return PMW_EINVAL;
   switch (nt) {
-    case 256: return launch_x<256>(a, g, st);
-    case 448: return launch_x<448>(a, g, st);
-    case 512: return launch_x<512>(a, g, st);
+    case 256: return launch_x<256>(a, g, st, members);
+    case 448: return launch_x<448>(a, g, st, members);
+    case 512: return launch_x<512>(a, g, st, members);
     default: return PMW_EINVAL;
   }
 }
 
-int launch_dir_z(const DirArgs& a, Grid g, int seg, hipStream_t st) {
+int launch_dir_z(const DirArgs& a, Grid g, int seg, hipStream_t st, int members) {
+  if (members < 1 || members > 65535 || (members > 1 && a.mstride <= 0)) return PMW_EINVAL;
   switch (seg) {
-    case 8: return launch_z<8>(a, g, st);
-    case 16: return launch_z<16>(a, g, st);
-    case 24: return launch_z<24>(a, g, st);
-    case 32: return launch_z<32>(a, g, st);
-    case 48: return launch_z<48>(a, g, st);
-    case 64: return launch_z<64>(a, g, st);
+    case 8: return launch_z<8>(a, g, st, members);
+    case 16: return launch_z<16>(a, g, st, members);
+    case 24: return launch_z<24>(a, g, st, members);
+    case 32: return launch_z<32>(a, g, st, members);
+    case 48: return launch_z<48>(a, g, st, members);
+    case 64: return launch_z<64>(a, g, st, members);
     default: return PMW_EINVAL;
   }
 }

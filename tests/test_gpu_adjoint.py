@@ -33,7 +33,8 @@ def _perturbed_interior(f, seed):
 
 
 @pytest.mark.parametrize("direction", [0, 1])
-@pytest.mark.parametrize("ic,nx,nz", [("thermal", 12, 7), ("collision", 40, 20)])
+@pytest.mark.parametrize("ic,nx,nz", [("thermal", 12, 7), ("collision", 40, 20),
+                                      ("gravity", 300, 21), ("thermal", 2, 2)])
 def test_stage_adjoint_matches_autograd(direction, ic, nx, nz):
     """lam_f = a * T'(f)^T lam for one stage equals torch autograd of
     <lam, a * T(f)> (halo, reconstruction, flux, tendency, gravity)."""
@@ -56,9 +57,8 @@ def test_stage_adjoint_matches_autograd(direction, ic, nx, nz):
     lam = torch.zeros_like(fs)
     lam[:, 2:-2, 2:-2] = lam_i
     lout = torch.zeros_like(fs)
-    fw = torch.zeros(8 * (nz + 1) * (nx + 1), dtype=torch.float64, device="cuda")
     _lib.call("pmw_adjoint_stage", direction, _lib.ptr(fs), _lib.ptr(lam), a, None, None,
-              None, _lib.ptr(lout), _lib.ptr(fw), _lib.ptr(f.hy_dens_cell),
+              None, _lib.ptr(lout), _lib.ptr(f.hy_dens_cell),
               _lib.ptr(f.hy_dens_theta_cell), _lib.ptr(f.hy_dens_int),
               _lib.ptr(f.hy_dens_theta_int), nx, nz, d, hv, _lib.stream_of())
     torch.cuda.synchronize()
@@ -96,6 +96,84 @@ def test_gradient_matches_torch_autograd(ic, nx, nz, steps):
     assert abs(loss - lref) <= 1e-12 * abs(lref)
     r = rel_l2(grad.cpu().numpy(), gref.cpu().numpy())
     assert r <= 1e-11, f"gradient rel L2 {r:.3e}"
+
+
+def test_tape_forward_bitwise_equals_evolve():
+    """Every state slot the tape keeps equals the state the fused evolve
+    passes through (two members, both direction orders)."""
+    import ctypes
+    from pyminiweather_amd import _lib
+    from pyminiweather_amd.adjoint import AdjointSolver
+    import pyminiweather_amd.solve as S
+    p = O.make_params(70, 33, "collision", cfl=1.5, max_speed=450.0)
+    f = _fields(p)
+    solver = AdjointSolver(p, f, 3, checkpoint_every=3, members=2)
+    s0 = torch.stack([f.state, f.state.clone()])
+    s0[1, :, 2:-2, 2:-2] += _perturbed_interior(f, 4) - f.state[:, 2:-2, 2:-2]
+    solver._tape_run(s0, 1, 3, store=True)
+    tape = solver._tape
+    for m in range(2):
+        st, tmp, work = s0[m].clone(), s0[m].clone(), torch.zeros_like(s0[m])
+        for k in range(3):
+            flag = ctypes.c_int((1 + k) % 2)
+            tune = (ctypes.c_int * 3)(*solver.tuning)
+            _lib.call("pmw_evolve_ex", _lib.ptr(st), _lib.ptr(tmp), _lib.ptr(work),
+                      _lib.ptr(f.hy_dens_cell), _lib.ptr(f.hy_dens_theta_cell),
+                      _lib.ptr(f.hy_dens_int), _lib.ptr(f.hy_dens_theta_int),
+                      _lib.ptr(f.hy_pressure_int), None, 0, None, f.nx, f.nz, p["dx"], p["dz"],
+                      p["dt"], S.hv_coeff(p["dx"], p["dt"]), S.hv_coeff(p["dz"], p["dt"]),
+                      ctypes.byref(flag), 1, tune, _lib.stream_of(), None)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(tape[6 * k + 3, m, :, 2:-2, 2:-2].cpu().numpy(),
+                                          work[:, 2:-2, 2:-2].cpu().numpy())
+            np.testing.assert_array_equal(tape[6 * k + 5, m, :, 2:-2, 2:-2].cpu().numpy(),
+                                          tmp[:, 2:-2, 2:-2].cpu().numpy())
+            np.testing.assert_array_equal(tape[6 * k + 6, m, :, 2:-2, 2:-2].cpu().numpy(),
+                                          st[:, 2:-2, 2:-2].cpu().numpy())
+
+
+def test_tape_stage1_slots_match_rk_stage():
+    """Stage-1 slots hold S + dt/3 T(S) as the per-stage kernel computes it."""
+    from pyminiweather_amd import _lib
+    from pyminiweather_amd.adjoint import AdjointSolver
+    import pyminiweather_amd.solve as S
+    p = O.make_params(50, 24, "thermal", cfl=1.5, max_speed=450.0)
+    f = _fields(p)
+    solver = AdjointSolver(p, f, 1, members=1)
+    s0 = f.state[None].clone()
+    s0[0, :, 2:-2, 2:-2] = _perturbed_interior(f, 8)
+    solver._tape_run(s0, 0, 1, store=True)          # z first
+    tape = solver._tape
+    for slot, start, direction in ((1, 0, 1), (4, 3, 0)):
+        init = tape[start, 0].clone()
+        out = torch.zeros_like(init)
+        d = p["dx"] if direction == 0 else p["dz"]
+        _lib.call("pmw_rk_stage", direction, _lib.ptr(init), _lib.ptr(init), _lib.ptr(out),
+                  _lib.ptr(f.hy_dens_cell), _lib.ptr(f.hy_dens_theta_cell),
+                  _lib.ptr(f.hy_dens_int), _lib.ptr(f.hy_dens_theta_int),
+                  _lib.ptr(f.hy_pressure_int), None, 0, None, f.nx, f.nz, d, p["dt"] / 3,
+                  S.hv_coeff(d, p["dt"]), _lib.stream_of())
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(tape[slot, 0, :, 2:-2, 2:-2].cpu().numpy(),
+                                      out[:, 2:-2, 2:-2].cpu().numpy())
+
+
+def test_batched_members_bitwise_equal_single():
+    """A batched solve of 3 members gives each member the gradient its own
+    single-member solve gives, bit for bit."""
+    from pyminiweather_amd.adjoint import AdjointSolver
+    p = O.make_params(36, 18, "thermal", cfl=1.5, max_speed=450.0)
+    f = _fields(p)
+    g = torch.Generator(device="cuda").manual_seed(21)
+    theta = torch.randn((3, 18, 36), dtype=torch.float64, device="cuda", generator=g)
+    target = torch.randn((3, 18, 36), dtype=torch.float64, device="cuda", generator=g)
+    batched = AdjointSolver(p, f, 7, members=3)
+    losses, grads = batched.loss_and_grad(theta, target)
+    single = AdjointSolver(p, f, 7)
+    for m in range(3):
+        lm, gm = single.loss_and_grad(theta[m], target[m])
+        assert lm == float(losses[m])
+        np.testing.assert_array_equal(gm.cpu().numpy(), grads[m].cpu().numpy())
 
 
 def test_checkpoint_interval_does_not_change_gradient():
@@ -143,8 +221,7 @@ def _config3():
     w = solver.row_weight()[:, None]
     theta0 = f.state[3, 2:-2, 2:-2] / w          # the thermal IC as a theta' field
     noise = torch.from_numpy(np.random.default_rng(1).normal(0.0, 0.1, (200, 400))).cuda()
-    target, _ = solver.forward(solver.state_from_theta(theta0 + noise))
-    target = target[3, 2:-2, 2:-2].clone()
+    target = solver.final_rhot(theta0 + noise)
     loss, grad = solver.loss_and_grad(theta0, target)
     return solver, theta0, target, loss, grad
 
@@ -197,3 +274,27 @@ def test_config3_gradient_matches_central_differences():
         if eps == 1e-3:
             assert rel <= 1e-6, report
     print("; ".join(report))
+
+
+def test_config4_batched_assimilation_reduces_every_loss():
+    """64 members on 400x200, 400 steps: member m's target is the forward
+    run from the thermal IC plus N(0, 0.1 K) theta' noise drawn with numpy
+    default_rng(2 + m); starting from the thermal IC, 20 steepest-descent
+    iterations with batched forward + checkpointed adjoint lower every
+    member's loss, each iteration's loss is below the previous one, and
+    the final loss is under 5% of the first (measured: 1.5-2.1%)."""
+    from pyminiweather_amd.adjoint import AdjointSolver
+    from pyminiweather_amd.ensemble import member_params
+    p = member_params(400, 200)
+    f = _fields(p)
+    M = 64
+    solver = AdjointSolver(p, f, 400, members=M)
+    w = solver.row_weight()[:, None]
+    theta0 = (f.state[3, 2:-2, 2:-2] / w).expand(M, 200, 400).clone()
+    noise = np.stack([np.random.default_rng(2 + m).normal(0.0, 0.1, (200, 400))
+                      for m in range(M)])
+    target = solver.final_rhot(theta0 + torch.from_numpy(noise).cuda())
+    _, hist = solver.assimilate(theta0, target, iters=20)
+    assert hist.shape == (21, M)
+    assert np.all(np.diff(hist, axis=0) < 0)
+    assert np.all(hist[-1] < 0.05 * hist[0])
