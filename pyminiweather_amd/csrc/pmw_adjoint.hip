@@ -42,22 +42,32 @@ constexpr double kD3[4] = {-1.0, 3.0, -3.0, 1.0};
 constexpr int kAdjXThreads = 256;          // x: faces per block
 constexpr int kAdjXCells = kAdjXThreads - 3;  // x: cells per block
 constexpr int kAdjZRows = 8;               // z: cell rows per block (64 columns)
+constexpr int kAdjZWaves = 8;              // z: waves per block
 
 __device__ __forceinline__ int wrapc(int c, int n) {
   int w = c % n;
   return w < 0 ? w + n : w;
 }
 
+// Primitive variables of a face and dp/dq (q = rho*theta, p = C0 q^gamma):
+// one division and one pow per face.
+__device__ __forceinline__ void face_prims(const double val[4], double hyr, double hyt,
+                                           double& u, double& w, double& t, double& dp) {
+  const double rho = val[0] + hyr;
+  const double ir = 1.0 / rho;
+  const double q = val[3] + hyt;
+  u = val[1] * ir;
+  w = val[2] * ir;
+  t = q * ir;
+  dp = (kGamma * kC0) * pow(q, kGamma - 1.0);
+}
+
 // Transposed x-flux Jacobian: cotangents of (vals, d3) from lamF.
 __device__ __forceinline__ void adj_flux_x(const double val[4], double hyr, double hyt,
                                            double hv, const double lf[4], double lv[4],
                                            double ld[4]) {
-  const double rho = val[0] + hyr;
-  const double u = val[1] / rho;
-  const double w = val[2] / rho;
-  const double t = (val[3] + hyt) / rho;
-  const double q = rho * t;
-  const double dp = kGamma * (kC0 * pow(q, kGamma)) / q;  // dp/dq
+  double u, w, t, dp;
+  face_prims(val, hyr, hyt, u, w, t, dp);
   lv[0] = -(u * u) * lf[1] - (u * w) * lf[2] - (u * t) * lf[3];
   lv[1] = lf[0] + 2.0 * u * lf[1] + w * lf[2] + t * lf[3];
   lv[2] = u * lf[2];
@@ -70,12 +80,8 @@ __device__ __forceinline__ void adj_flux_x(const double val[4], double hyr, doub
 __device__ __forceinline__ void adj_flux_z(const double val[4], double hyr, double hyt,
                                            double hv, bool wall, const double lf[4],
                                            double lv[4], double ld[4]) {
-  const double rho = val[0] + hyr;
-  const double u = val[1] / rho;
-  const double w = val[2] / rho;
-  const double t = (val[3] + hyt) / rho;
-  const double q = rho * t;
-  const double dp = kGamma * (kC0 * pow(q, kGamma)) / q;
+  double u, w, t, dp;
+  face_prims(val, hyr, hyt, u, w, t, dp);
   if (wall) {
     lv[0] = lv[1] = lv[2] = 0.0;
     lv[3] = dp * lf[2];
@@ -105,7 +111,9 @@ struct AdjArgs {
   const double* hyt_cell;
   const double* hyr_int;
   const double* hyt_int;
-  double scale, d, hv;
+  double scale;   // stage dt
+  double sd;      // scale / d
+  double hv;
   long long mstride;
 };
 
@@ -142,20 +150,25 @@ __global__ __launch_bounds__(kAdjXThreads) void k_adj_x(AdjArgs a, Grid g, int n
   const int prow = k + kHs;
   {
     const int face = c0 - 1 + t;  // reads cells face-2 .. face+1
+    int col[4];                   // padded columns of those cells
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = face - 2 + j;
+      col[j] = (c >= 0 && c < nx ? c : wrapc(c, nx)) + kHs;
+    }
     double val[4], d3[4], lf[4], lv[4], ld[4];
 #pragma unroll
     for (int v = 0; v < kNv; ++v) {
       double s[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) s[j] = a.f[g.at(v, prow, wrapc(face - 2 + j, nx) + kHs)];
+      for (int j = 0; j < 4; ++j) s[j] = a.f[g.at(v, prow, col[j])];
       recon4(s[0], s[1], s[2], s[3], val[v], d3[v]);
     }
-    // T(i) = (F(i) - F(i+1)) / dx: the face is F(i) of cell `face` and
-    // F(i+1) of cell face-1
-    const int cr = wrapc(face, nx) + kHs, cl = wrapc(face - 1, nx) + kHs;
+    // T(i) = (F(i) - F(i+1)) / dx: the face is F(i) of cell `face` (col[2])
+    // and F(i+1) of cell face-1 (col[1])
 #pragma unroll
     for (int v = 0; v < kNv; ++v)
-      lf[v] = a.scale * (a.lam[g.at(v, prow, cr)] - a.lam[g.at(v, prow, cl)]) / a.d;
+      lf[v] = a.sd * (a.lam[g.at(v, prow, col[2])] - a.lam[g.at(v, prow, col[1])]);
     adj_flux_x(val, a.hy_cell[prow], a.hyt_cell[prow], a.hv, lf, lv, ld);
 #pragma unroll
     for (int v = 0; v < kNv; ++v) {
@@ -204,7 +217,7 @@ __device__ __forceinline__ void gather_z_row(const double (*fw)[R + 3][64], int 
 }
 
 template <int R>
-__global__ __launch_bounds__(256) void k_adj_z(AdjArgs a, Grid g, int ncol) {
+__global__ __launch_bounds__(64 * kAdjZWaves) void k_adj_z(AdjArgs a, Grid g, int ncol) {
   constexpr int NF = R + 3;
   __shared__ double fw[8][NF][64];
   adj_offset(a);
@@ -213,7 +226,7 @@ __global__ __launch_bounds__(256) void k_adj_z(AdjArgs a, Grid g, int ncol) {
   const int i = (blockIdx.x % ncol) * 64 + lane, r0 = (blockIdx.x / ncol) * R;
   const int pc = i + kHs;
   if (i < nx) {
-    for (int s = wy; s < NF; s += 4) {
+    for (int s = wy; s < NF; s += kAdjZWaves) {
       const int face = r0 - 1 + s;
       if (face < 0 || face > nz) continue;
       double val[4], d3[4], lf[4], lv[4], ld[4];
@@ -228,7 +241,7 @@ __global__ __launch_bounds__(256) void k_adj_z(AdjArgs a, Grid g, int ncol) {
       for (int v = 0; v < kNv; ++v) {
         const double above = face < nz ? a.lam[g.at(v, face + kHs, pc)] : 0.0;
         const double below = face > 0 ? a.lam[g.at(v, face - 1 + kHs, pc)] : 0.0;
-        lf[v] = a.scale * (above - below) / a.d;
+        lf[v] = a.sd * (above - below);
       }
       adj_flux_z(val, a.hyr_int[face], a.hyt_int[face], a.hv, face == 0 || face == nz, lf, lv,
                  ld);
@@ -241,7 +254,7 @@ __global__ __launch_bounds__(256) void k_adj_z(AdjArgs a, Grid g, int ncol) {
   }
   __syncthreads();
   if (i >= nx) return;
-  for (int rr = wy; rr < R; rr += 4) {
+  for (int rr = wy; rr < R; rr += kAdjZWaves) {
     const int k = r0 + rr;
     if (k >= nz) break;
     double l[4], h[4];
@@ -269,14 +282,15 @@ int adj_stage(int dir, const double* f, const double* lam, double scale, const d
               const double* hyt_int, Grid g, double d, double hv, int members,
               long long mstride, hipStream_t st) {
   AdjArgs a{f, lam, acc1, acc2, acc3, lam_out, hy_cell, hyt_cell, hyr_int, hyt_int,
-            scale, d, hv, mstride};
+            scale, scale / d, hv, mstride};
   if (dir == 0) {
     const int ntiles = (g.nx + kAdjXCells - 1) / kAdjXCells;
     k_adj_x<<<dim3((unsigned)(ntiles * g.nz), members), kAdjXThreads, 0, st>>>(a, g, ntiles);
   } else {
     const int ncol = (g.nx + 63) / 64;
     const int nrt = (g.nz + kAdjZRows - 1) / kAdjZRows;
-    k_adj_z<kAdjZRows><<<dim3((unsigned)(ncol * nrt), members), 256, 0, st>>>(a, g, ncol);
+    k_adj_z<kAdjZRows><<<dim3((unsigned)(ncol * nrt), members), 64 * kAdjZWaves, 0, st>>>(
+        a, g, ncol);
   }
   PMW_CHECK_LAUNCH();
   return PMW_OK;
