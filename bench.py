@@ -39,7 +39,7 @@ def workload_params(nx=2000, nz=1000):
             "dt": min(2.0e4 / nx, 1.0e4 / nz) * 1.5 / 450.0}
 
 
-PROFILE_SUMMARY = os.path.join(REPO, "profiles", "stage_kernels_collision_2000x1000.txt")
+PROFILE_SUMMARY = os.path.join(REPO, "profiles", "fused_direction_kernels_collision_2000x1000.txt")
 
 
 def traffic_from_profile(kernel_prefix):
@@ -145,6 +145,115 @@ def bench_ensemble(nsteps, loop_steps, autotune):
             "tuning": list(ens.tuning)}
 
 
+def _thermal_400x200():
+    from pyminiweather_amd.data import initialize_fields
+    from pyminiweather_amd.ensemble import member_params
+    from pyminiweather_amd.ics import init
+    from pyminiweather_amd.mesh import MeshData
+    p = member_params(400, 200)
+    f = initialize_fields(p)
+    init(f, p, MeshData(p))
+    return p, f
+
+
+def _timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = fn()
+    torch.cuda.synchronize()
+    return r, time.perf_counter() - t0
+
+
+def bench_adjoint(nsteps=400, fd_cells=8):
+    """Differentiation form, single grid: L = 1/2 ||(rho*theta)'(T) -
+    target||^2 on the 400x200 thermal bubble after `nsteps` RK3 steps
+    (target: forward run from the thermal IC + N(0, 0.1 K) theta' noise,
+    numpy default_rng(1)); gradient w.r.t. the initial theta' with sqrt(N)
+    checkpoints. Times the checkpointed forward sweep and the backward sweep
+    (segment recompute into the tape + adjoint kernels), and checks
+    `fd_cells` random cells against central differences."""
+    import numpy as np
+    import torch
+    from pyminiweather_amd.adjoint import AdjointSolver
+    p, f = _thermal_400x200()
+    solver = AdjointSolver(p, f, nsteps)
+    w = solver.row_weight()[:, None]
+    theta0 = (f.state[3, 2:-2, 2:-2] / w).clone()
+    noise = torch.from_numpy(np.random.default_rng(1).normal(0.0, 0.1, (200, 400))).cuda()
+    target = solver.final_rhot(theta0 + noise)
+    solver.loss_and_grad(theta0, target)  # warm-up
+    s0 = solver.state_from_theta(theta0)
+    (final, ckpts), t_fwd = _timed(lambda: solver.forward(s0))
+    lam = torch.zeros_like(final)
+    lam[:, 3, 2:-2, 2:-2] = final[:, 3, 2:-2, 2:-2] - target
+    lam0, t_bwd = _timed(lambda: solver.backward(lam, ckpts))
+    grad = lam0[0, 3, 2:-2, 2:-2] * w
+    cells = 400 * 200 * nsteps
+    fd = {}
+    if fd_cells:
+        rng = np.random.default_rng(7)
+        picks = [(int(rng.integers(200)), int(rng.integers(400))) for _ in range(fd_cells)]
+        g = np.array([float(grad[k, i]) for k, i in picks])
+        for eps in (1e-6, 1e-3):
+            q = []
+            for k, i in picks:
+                e = torch.zeros_like(theta0)
+                e[k, i] = eps
+                q.append((solver.loss(theta0 + e, target) - solver.loss(theta0 - e, target))
+                         / (2 * eps))
+            fd[f"eps={eps:g}"] = float(np.linalg.norm(np.array(q) - g) / np.linalg.norm(g))
+    return {"workload": f"thermal 400x200, {nsteps} RK3 steps, gradient of the final "
+                        "(rho*theta)' misfit w.r.t. the initial theta'",
+            "checkpoint_every": solver.every,
+            "value": cells / (t_fwd + t_bwd), "unit": "fwd+adjoint cell-steps/s",
+            "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
+            "adjoint_cost_vs_forward": t_bwd / t_fwd,
+            "fd_rel_l2": fd, "fd_cells": fd_cells,
+            "tuning": list(solver.tuning)}
+
+
+def bench_batched_adjoint(members=64, nsteps=400, iters=20):
+    """Batched assimilation: `members` 400x200 members, member m's target the
+    forward run from the thermal IC + N(0, 0.1 K) theta' noise (numpy
+    default_rng(2 + m)); `iters` steepest-descent iterations on the initial
+    theta', forward and checkpointed adjoint batched in one launch per
+    direction (forward) / stage (adjoint) for all members."""
+    import numpy as np
+    import torch
+    from pyminiweather_amd.adjoint import AdjointSolver
+    p, f = _thermal_400x200()
+    solver = AdjointSolver(p, f, nsteps, members=members)
+    w = solver.row_weight()[:, None]
+    theta0 = (f.state[3, 2:-2, 2:-2] / w).expand(members, 200, 400).clone()
+    noise = np.stack([np.random.default_rng(2 + m).normal(0.0, 0.1, (200, 400))
+                      for m in range(members)])
+    target = solver.final_rhot(theta0 + torch.from_numpy(noise).cuda())
+    s0 = solver.state_from_theta(theta0)
+    (final, ckpts), t_fwd = _timed(lambda: solver.forward(s0))
+    lam = torch.zeros_like(final)
+    lam[:, 3, 2:-2, 2:-2] = final[:, 3, 2:-2, 2:-2] - target
+    _, t_bwd = _timed(lambda: solver.backward(lam, ckpts))
+    cells = members * 400 * 200 * nsteps
+    out = {"workload": f"{members} members 400x200, {nsteps} RK3 steps, per-member "
+                       "seeded targets (seeds 2..), steepest descent on theta'",
+           "members": members, "value": cells / (t_fwd + t_bwd),
+           "unit": "fwd+adjoint member-cell-steps/s",
+           "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
+           "adjoint_cost_vs_forward": t_bwd / t_fwd,
+           "launch": "one launch per direction (forward) and per RK stage (adjoint) "
+                     "for all members"}
+    if iters:
+        (_, hist), t_all = _timed(lambda: solver.assimilate(theta0, target, iters=iters))
+        red = hist[-1] / hist[0]
+        out.update({"iterations": iters, "assimilation_s": t_all,
+                    "final_over_initial_loss": {"min": float(red.min()),
+                                                "median": float(np.median(red)),
+                                                "max": float(red.max())},
+                    "monotone_all_members": bool(np.all(np.diff(hist, axis=0) < 0))})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +267,11 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=300)
     ap.add_argument("--ensemble-loop-steps", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", type=str, default=None,
+                    help="mode,x_nt,z_seg for the main workload (skips its autotune)")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-members", type=int, default=64)
+    ap.add_argument("--assim-iters", type=int, default=20)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 512 MiB device copies (scales the rocprof "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -195,7 +309,9 @@ def main():
     S.reverse_direction = False
     m0, e0 = compute_stats(p, f)
     tune_table = None
-    if not args.no_autotune:
+    if args.tuning:
+        S.set_tuning(nx, nz, tuple(int(v) for v in args.tuning.split(",")))
+    elif not args.no_autotune:
         from pyminiweather_amd.autotune import tune_grid
         _, tune_table = tune_grid(p, f)
 
@@ -275,6 +391,10 @@ def main():
     if not args.no_ensemble:
         out["batched"] = bench_ensemble(args.ensemble_steps, args.ensemble_loop_steps,
                                         not args.no_autotune)
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint()
+        out["batched_adjoint"] = bench_batched_adjoint(args.adjoint_members,
+                                                       iters=args.assim_iters)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(nx, nz)
     print(json.dumps(out), flush=True)

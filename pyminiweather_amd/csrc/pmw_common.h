@@ -49,6 +49,17 @@ struct Grid {
   }
 };
 
+// a / b from y = RN(1/b): q = RN(a*y) is within an ulp of a/b and the
+// residual a - q*b is exact under fma, so q + r*y rounds to the IEEE
+// quotient (Markstein's correction; valid while a/b, a*y and the residual
+// neither overflow nor underflow, as for the fluxes and densities here).
+// Three instructions instead of a full division.
+__device__ __forceinline__ double div_rn(double a, double b, double y) {
+  const double q = a * y;
+  const double r = fma(-q, b, a);
+  return fma(r, y, q);
+}
+
 // 4-tap reconstruction at one face: value and 3rd difference, accumulated
 // left to right like scipy.signal.convolve's direct method.
 __device__ __forceinline__ void recon4(double s0, double s1, double s2,
@@ -63,9 +74,10 @@ __device__ __forceinline__ void flux_x_face(const double val[4],
                                             double hyt, double hv,
                                             double F[4]) {
   const double rho = val[0] + hyr;
-  const double u = val[1] / rho;
-  const double w = val[2] / rho;
-  const double t = (val[3] + hyt) / rho;
+  const double ir = 1.0 / rho;
+  const double u = div_rn(val[1], rho, ir);
+  const double w = div_rn(val[2], rho, ir);
+  const double t = div_rn(val[3] + hyt, rho, ir);
   const double p = kC0 * pow(rho * t, kGamma);
   const double ru = rho * u;
   F[0] = ru - hv * d3[0];
@@ -81,9 +93,10 @@ __device__ __forceinline__ void flux_z_face(const double val[4], double d3[4],
                                             double hyp, double hv, bool wall,
                                             double G[4]) {
   const double rho = val[0] + hyr;
-  const double u = val[1] / rho;
-  double w = val[2] / rho;
-  const double t = (val[3] + hyt) / rho;
+  const double ir = 1.0 / rho;
+  const double u = div_rn(val[1], rho, ir);
+  double w = div_rn(val[2], rho, ir);
+  const double t = div_rn(val[3] + hyt, rho, ir);
   const double p = kC0 * pow(rho * t, kGamma) - hyp;
   if (wall) {
     w = 0.0;
@@ -97,8 +110,9 @@ __device__ __forceinline__ void flux_z_face(const double val[4], double d3[4],
 }
 
 // Tendency from two face fluxes (solve/interpolate.py:209-215, 239-245).
-__device__ __forceinline__ double tend_of(double f_lo, double f_hi, double d) {
-  return (-(f_hi - f_lo)) / d;
+// id = RN(1/d).
+__device__ __forceinline__ double tend_of(double f_lo, double f_hi, double d, double id) {
+  return div_rn(-(f_hi - f_lo), d, id);
 }
 
 // Value the reference's set_bc_z leaves in z-halo row `prow` (0, 1, nz+2 or

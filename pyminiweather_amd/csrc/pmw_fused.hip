@@ -82,6 +82,7 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
   }
   const double hyr = a.hy_cell[prow], hyt = a.hyt_cell[prow];
   const double src = a.src_w ? a.src_w[srow + cw] : 0.0;
+  const double id = 1.0 / a.d;
   __syncthreads();
   // ---- stage 1: faces 2..NT-2, cells 2..NT-3, S -> s1
   {
@@ -100,7 +101,7 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
   if (t >= 2 && t <= NT - 3) {
     double td[4];
 #pragma unroll
-    for (int v = 0; v < kNv; ++v) td[v] = tend_of(fl[v][t], fl[v][t + 1], a.d);
+    for (int v = 0; v < kNv; ++v) td[v] = tend_of(fl[v][t], fl[v][t + 1], a.d, id);
     if (a.src_w) td[2] = td[2] + src;
 #pragma unroll
     for (int v = 0; v < kNv; ++v) s1[v][t] = s0[v][t] + a.dt1 * td[v];
@@ -128,7 +129,7 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
     double q[4];
 #pragma unroll
     for (int v = 0; v < kNv; ++v) {
-      const double td = tend_of(fl[v][t], fl[v][t + 1], a.d);
+      const double td = tend_of(fl[v][t], fl[v][t + 1], a.d, id);
       q[v] = td;
     }
     if (a.src_w) q[2] = q[2] + src;
@@ -162,7 +163,7 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
   if (t >= 6 && t <= NT - 7 && cell < nx) {
     double td[4];
 #pragma unroll
-    for (int v = 0; v < kNv; ++v) td[v] = tend_of(fl[v][t], fl[v][t + 1], a.d);
+    for (int v = 0; v < kNv; ++v) td[v] = tend_of(fl[v][t], fl[v][t + 1], a.d, id);
     if (a.src_w) td[2] = td[2] + src;
 #pragma unroll
     for (int v = 0; v < kNv; ++v) a.out[g.at(v, prow, cell + kHs)] = s0[v][t] + a.dt3 * td[v];
@@ -268,9 +269,10 @@ __device__ __forceinline__ bool z_stage_step(double w[4][4], double F[4], const 
     flux_z_face(v4, d3, a.hyr_int[f], a.hyt_int[f], a.hyp_int[f], a.hv, f == 0 || f == nz, G);
     const int c = f - 1;  // window row 1
     if (c >= lo) {
+      const double id = 1.0 / a.d;
       double td[4];
 #pragma unroll
-      for (int v = 0; v < kNv; ++v) td[v] = tend_of(F[v], G[v], a.d);
+      for (int v = 0; v < kNv; ++v) td[v] = tend_of(F[v], G[v], a.d, id);
       td[2] = td[2] - w[S1][0] * kGrav;
       if (a.src_w) td[2] = td[2] + a.src_w[(long long)c * g.nx + i];
       if (stage == 0) {

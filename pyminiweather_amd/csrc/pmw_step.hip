@@ -161,7 +161,7 @@ __global__ void k_tend_x(const double* __restrict__ flux, double* __restrict__ t
   const int v = (int)(idx / pt);
   const long long pf = (long long)(g.nz + 1) * (g.nx + 1);
   const double* f = flux + v * pf + (long long)k * (g.nx + 1) + i;
-  tend[idx] = tend_of(f[0], f[1], dx);
+  tend[idx] = tend_of(f[0], f[1], dx, 1.0 / dx);
 }
 
 __global__ void k_tend_z(const double* __restrict__ flux, const double* __restrict__ sf,
@@ -175,7 +175,7 @@ __global__ void k_tend_z(const double* __restrict__ flux, const double* __restri
   const int v = (int)(idx / pt);
   const long long pf = (long long)(g.nz + 1) * (g.nx + 1);
   const double* f = flux + v * pf + (long long)k * (g.nx + 1) + i;
-  double t = tend_of(f[0], f[g.nx + 1], dz);
+  double t = tend_of(f[0], f[g.nx + 1], dz, 1.0 / dz);
   if (v == 2) t = t - sf[g.at(0, k + kHs, i + kHs)] * kGrav;
   tend[idx] = t;
 }
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(kXThreads) void k_stage_x(
   if (t < kXCells && i < nx) {
     double td[4];
 #pragma unroll
-    for (int v = 0; v < kNv; ++v) td[v] = tend_of(sf[v][t], sf[v][t + 1], dx);
+    for (int v = 0; v < kNv; ++v) td[v] = tend_of(sf[v][t], sf[v][t + 1], dx, 1.0 / dx);
     const long long cell = (long long)k * nx + i;
     if (src_w) td[2] = td[2] + src_w[cell];
     if (MODE == 0) {
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kZLanes* kZWaves) void k_stage_z(
       // cell k-1; its forcing rho' is interior row k-1 = window row 1
       double td[4];
 #pragma unroll
-      for (int v = 0; v < kNv; ++v) td[v] = tend_of(Gp[v], G[v], dz);
+      for (int v = 0; v < kNv; ++v) td[v] = tend_of(Gp[v], G[v], dz, 1.0 / dz);
       td[2] = td[2] - w1[0] * kGrav;
       const long long cell = (long long)(k - 1) * nx + i;
       if (src_w) td[2] = td[2] + src_w[cell];
