@@ -19,7 +19,9 @@ from . import solve as S
 from ._lib import ptr, stream_of
 
 X_BLOCKS = (256, 448, 512)
-Z_SEGMENTS = (8, 16, 24, 32, 48, 64)
+# z: rows per marching segment, or an LDS tile code (8064: 8 columns x 64
+# rows, 4128: 4 columns x 128 rows)
+Z_SEGMENTS = (8, 16, 24, 32, 48, 64, 8064, 4128)
 
 
 def _time_grid(params, fields, tuning, steps, bufs):

@@ -225,7 +225,7 @@ template <int PH, int stage>
 __device__ __forceinline__ bool z_stage_step(double w[4][4], double F[4], const DirArgs& a,
                                              const Grid& g, int r, int lo,
                                              int hi, int pc, int i, const double* lds_row,
-                                             double dts, double res[4]) {
+                                             const double cur[4], double dts, double res[4]) {
   constexpr int S0 = (PH + 1) & 3, S1 = (PH + 2) & 3, S2 = (PH + 3) & 3, S3 = PH & 3;
   const int nz = g.nz;
   const double* hy = a.hy_cell;
@@ -236,7 +236,7 @@ __device__ __forceinline__ bool z_stage_step(double w[4][4], double F[4], const 
     if (r >= 0 && r < nz) {
       if (stage == 0) {
 #pragma unroll
-        for (int v = 0; v < kNv; ++v) nv[v] = a.in[g.at(v, r + kHs, pc)];
+        for (int v = 0; v < kNv; ++v) nv[v] = cur[v];  // row r, loaded a step ahead
       } else {
 #pragma unroll
         for (int v = 0; v < kNv; ++v) nv[v] = lds_row[v * 64];
@@ -261,6 +261,14 @@ __device__ __forceinline__ bool z_stage_step(double w[4][4], double F[4], const 
   }
   const int f = r - 1;
   bool done = false;
+  // RK init of the cell this step may finish (row r-2; clamped so the load
+  // is always in bounds), issued before the face so its latency overlaps it
+  double iv[4];
+  if (stage > 0) {
+    const int cc = min(max(r - 2, 0), nz - 1);
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) iv[v] = a.in[g.at(v, cc + kHs, pc)];
+  }
   if (f >= lo && f <= hi) {
     double v4[4], d3[4], G[4];
 #pragma unroll
@@ -280,7 +288,7 @@ __device__ __forceinline__ bool z_stage_step(double w[4][4], double F[4], const 
         for (int v = 0; v < kNv; ++v) res[v] = w[S1][v] + dts * td[v];
       } else {
 #pragma unroll
-        for (int v = 0; v < kNv; ++v) res[v] = a.in[g.at(v, c + kHs, pc)] + dts * td[v];
+        for (int v = 0; v < kNv; ++v) res[v] = iv[v] + dts * td[v];
       }
       done = true;
     }
@@ -302,7 +310,7 @@ __device__ __forceinline__ void z_block_step(double w[4][4], double F[4], const 
                                              const Grid& g, double (*ring)[2][kNv][64],
                                              int j, int lo, int hi, int k0,
                                              int k1, int pc, int i, int lane, bool active,
-                                             double dts) {
+                                             const double cur[4], double dts) {
   const int nz = g.nz;
   const double* hy = a.hy_cell;
   const int r = j - 3 * stage;  // input row of this stage at step j
@@ -310,7 +318,8 @@ __device__ __forceinline__ void z_block_step(double w[4][4], double F[4], const 
     // rows of S1 / S2 come from the previous stage's ring (written one step ago)
     const double* lds_row = stage > 0 ? &ring[stage - 1][r & 1][0][lane] : nullptr;
     double res[4];
-    const bool done = z_stage_step<PH, stage>(w, F, a, g, r, lo, hi, pc, i, lds_row, dts, res);
+    const bool done =
+        z_stage_step<PH, stage>(w, F, a, g, r, lo, hi, pc, i, lds_row, cur, dts, res);
     if (stage == 0 && r == 0 && k0 == 0) {  // set_bc_z of stage 1 into st
       double h[4];
       wall_row(w[PH & 3], hy, kHs, 0, h);
@@ -369,6 +378,14 @@ __device__ __forceinline__ void z_march(const DirArgs& a, const Grid& g,
     for (int v = 0; v < kNv; ++v) w[p][v] = 0.0;
 #pragma unroll
   for (int v = 0; v < kNv; ++v) F[v] = 0.0;
+  // stage 0 reads S row j at step j: keep the next row's load in flight
+  const int nz = g.nz;
+  double pre[4] = {0.0, 0.0, 0.0, 0.0};
+  if (stage == 0 && active) {
+    const int rr = min(max(j0, 0), nz - 1);
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) pre[v] = a.in[g.at(v, rr + kHs, pc)];
+  }
 #pragma unroll 1
   for (int j = j0; j <= j1; ++j) {
     // slot p holds row r-3+p once the step's row has arrived in slot 3
@@ -376,7 +393,15 @@ __device__ __forceinline__ void z_march(const DirArgs& a, const Grid& g,
     for (int p = 0; p < 3; ++p)
 #pragma unroll
       for (int v = 0; v < kNv; ++v) w[p][v] = w[p + 1][v];
-    z_block_step<3, stage>(w, F, a, g, ring, j, lo, hi, k0, k1, pc, i, lane, active, dts);
+    double cur[4];
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) cur[v] = pre[v];
+    if (stage == 0 && active) {
+      const int rr = min(max(j + 1, 0), nz - 1);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) pre[v] = a.in[g.at(v, rr + kHs, pc)];
+    }
+    z_block_step<3, stage>(w, F, a, g, ring, j, lo, hi, k0, k1, pc, i, lane, active, cur, dts);
   }
 }
 
@@ -445,6 +470,140 @@ __global__ __launch_bounds__(192) void k_dir_z(DirArgs a, Grid g, int ncolblk, i
   if (logical >= nreal) return;  // whole block, before any barrier
   member_offset(a);
   dir_z_block<L>(a, g, logical % ncolblk, logical / ncolblk, ring);
+}
+
+// ------------------------------------------------------------- z tiles --
+// z direction as LDS tiles, the x kernel's scheme turned on its side: a block
+// of NT threads holds CW columns x NR = NT/CW rows (slot t = row*CW + col,
+// tile row j = logical row r0-6+j), runs the three stages in LDS (stage 1 on
+// tile rows 2..NR-3, stage 2 on 4..NR-5, stage 3 on the NR-12 output rows)
+// and rebuilds the wall halo rows of every stage state from its edge row, as
+// set_bc_z does between the stages (ics/bcs.py:64-148). Same operations in
+// the same order as the marching kernel, so results are bitwise equal.
+template <int CW, int NT>
+__device__ __forceinline__ void dir_zt_tile(const DirArgs& a, const Grid& g, int c0, int r0,
+                                            XTileLds<NT>& sm, bool corners) {
+  constexpr int NR = NT / CW, H = 6, C = NR - 2 * H;
+  auto& s0 = sm.s0;
+  auto& s1 = sm.s1;
+  auto& fl = sm.fl;
+  const int nx = g.nx, nz = g.nz, t = threadIdx.x;
+  const int j = t / CW, col = t % CW;
+  const int i = c0 + col;     // interior column
+  const int k = r0 - H + j;   // logical row of this slot
+  const bool colok = i < nx;
+  const int pc = i + kHs;
+  const double* hy = a.hy_cell;
+  const int plo = kHs, phi = nz + kHs - 1;  // padded edge rows
+  const bool wlo = k == -2 || k == -1, whi = k == nz || k == nz + 1;
+  const bool own = colok && k >= r0 && k < r0 + C && k < nz;
+  const bool own_lo = r0 == 0, own_hi = r0 <= nz - 1 && nz - 1 < r0 + C;
+  // slot of the edge row a wall slot is built from
+  const int edge_slot = wlo ? t - k * CW : t - (k - (nz - 1)) * CW;
+  {
+    double q[4] = {0.0, 0.0, 0.0, 0.0};
+    if (colok) {
+      if (k >= 0 && k < nz) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) q[v] = a.in[g.at(v, k + kHs, pc)];
+      } else if (wlo || whi) {
+        const int pe = wlo ? plo : phi;
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) q[v] = wall_halo(v, a.in[g.at(v, pe, pc)], hy[pe], hy[k + kHs]);
+        if ((wlo && own_lo) || (whi && own_hi)) {  // set_bc_z of stage 1 into st
+#pragma unroll
+          for (int v = 0; v < kNv; ++v) a.st[g.at(v, k + kHs, pc)] = q[v];
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) s0[v][t] = q[v];
+  }
+  if (corners && t < 8) {
+    // corner columns (padded 0, 1, nx+2, nx+3) of the halo rows of st, tmp
+    double* buf = t < 4 ? a.st : a.tmp;
+    const int qq = t & 3;
+    const int cc = qq < 2 ? qq : nx + qq;
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) {
+      const double e = buf[g.at(v, plo, cc)];
+      buf[g.at(v, 0, cc)] = wall_halo(v, e, hy[plo], hy[0]);
+      buf[g.at(v, 1, cc)] = wall_halo(v, e, hy[plo], hy[1]);
+      const double f = buf[g.at(v, phi, cc)];
+      buf[g.at(v, nz + 2, cc)] = wall_halo(v, f, hy[phi], hy[nz + 2]);
+      buf[g.at(v, nz + 3, cc)] = wall_halo(v, f, hy[phi], hy[nz + 3]);
+    }
+  }
+  const double id = 1.0 / a.d;
+  const double src = (a.src_w && colok && k >= 0 && k < nz) ? a.src_w[(long long)k * nx + i] : 0.0;
+  __syncthreads();
+#pragma unroll
+  for (int stage = 0; stage < 3; ++stage) {
+    double (*sin)[NT] = stage == 0 ? s0 : s1;
+    const int e = 2 + 2 * stage;
+    // faces: tile rows e .. NR-e (face k between logical rows k-1 and k)
+    if (colok && j >= e && j <= NR - e && k >= 0 && k <= nz) {
+      double v4[4], d3[4], G[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v)
+        recon4(sin[v][t - 2 * CW], sin[v][t - CW], sin[v][t], sin[v][t + CW], v4[v], d3[v]);
+      flux_z_face(v4, d3, a.hyr_int[k], a.hyt_int[k], a.hyp_int[k], a.hv, k == 0 || k == nz, G);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) fl[v][t] = G[v];
+    }
+    __syncthreads();
+    // cells: tile rows e .. NR-e-1
+    if (colok && j >= e && j <= NR - e - 1 && k >= 0 && k < nz) {
+      const double dts = stage == 0 ? a.dt1 : stage == 1 ? a.dt2 : a.dt3;
+      double td[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) td[v] = tend_of(fl[v][t], fl[v][t + CW], a.d, id);
+      td[2] = td[2] - sin[0][t] * kGrav;
+      if (a.src_w) td[2] = td[2] + src;
+      double q[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) q[v] = s0[v][t] + dts * td[v];
+      if (stage < 2) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) s1[v][t] = q[v];
+      }
+      if (own) {
+        double* dst = stage == 0 ? a.s1 : stage == 1 ? a.tmp : a.out;
+        if (dst) {
+#pragma unroll
+          for (int v = 0; v < kNv; ++v) dst[g.at(v, k + kHs, pc)] = q[v];
+        }
+      }
+    }
+    if (stage == 2) break;
+    __syncthreads();
+    // wall halo rows of the new stage state from its edge row
+    if (colok && (wlo || whi)) {
+      const int pe = wlo ? plo : phi;
+      double q[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) q[v] = wall_halo(v, s1[v][edge_slot], hy[pe], hy[k + kHs]);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) s1[v][t] = q[v];
+      if (stage == 1 && ((wlo && own_lo) || (whi && own_hi))) {  // set_bc_z of stage 3
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) a.tmp[g.at(v, k + kHs, pc)] = q[v];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int CW, int NT>
+__global__ __launch_bounds__(NT) void k_dir_zt(DirArgs a, Grid g, int ncolgrp, int nreal,
+                                               int per_xcd) {
+  constexpr int C = NT / CW - 12;
+  __shared__ XTileLds<NT> sm;
+  const int logical = xcd_logical(blockIdx.x, per_xcd);
+  if (logical >= nreal) return;  // whole block, before any barrier
+  member_offset(a);
+  dir_zt_tile<CW, NT>(a, g, (logical % ncolgrp) * CW, (logical / ncolgrp) * C, sm,
+                      logical == 0);
 }
 
 // ------------------------------------------------------------- ensemble --
@@ -541,6 +700,26 @@ __global__ __launch_bounds__(192) void k_ens_z(const MemberDev* __restrict__ m, 
   }
 }
 
+template <int CW, int NT>
+__global__ __launch_bounds__(NT) void k_ens_zt(const MemberDev* __restrict__ m, int n, int pass,
+                                               int total, int chunk, int per_xcd) {
+  constexpr int C = NT / CW - 12;
+  __shared__ XTileLds<NT> sm;
+  const int lb = xcd_logical(blockIdx.x, per_xcd);
+  const int first = lb * chunk;
+  const int last = min(total, first + chunk);
+  int mi = 0;
+  for (int item = first; item < last; ++item) {
+    mi = member_of<1>(m, n, item, mi);
+    const MemberDev& md = m[mi];
+    const DirArgs a = member_args(md, 1, pass);
+    const int local = item - md.z_start;
+    dir_zt_tile<CW, NT>(a, Grid(md.nx, md.nz), (local % md.z_ncol) * CW,
+                        (local / md.z_ncol) * C, sm, local == 0);
+    __syncthreads();  // the next tile rewrites the LDS this one last read
+  }
+}
+
 template <int NT>
 int launch_x(const DirArgs& a, Grid g, hipStream_t st, int members) {
   constexpr int C = NT - 12;
@@ -548,6 +727,18 @@ int launch_x(const DirArgs& a, Grid g, hipStream_t st, int members) {
   const long long nreal = (long long)ntiles * g.nz;
   const int per = (int)((nreal + kNumXcd - 1) / kNumXcd);
   k_dir_x<NT><<<dim3(per * kNumXcd, members), NT, 0, st>>>(a, g, ntiles, (int)nreal, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
+template <int CW, int NT>
+int launch_zt(const DirArgs& a, Grid g, hipStream_t st, int members) {
+  constexpr int C = NT / CW - 12;
+  const int ncolgrp = (g.nx + CW - 1) / CW;
+  const long long nreal = (long long)ncolgrp * ((g.nz + C - 1) / C);
+  const int per = (int)((nreal + kNumXcd - 1) / kNumXcd);
+  k_dir_zt<CW, NT><<<dim3(per * kNumXcd, members), NT, 0, st>>>(a, g, ncolgrp, (int)nreal,
+                                                                 per);
   PMW_CHECK_LAUNCH();
   return PMW_OK;
 }
@@ -568,8 +759,11 @@ int launch_z(const DirArgs& a, Grid g, hipStream_t st, int members) {
 namespace pmw {
 
 bool fused_x_tile_ok(int nt) { return nt == 256 || nt == 448 || nt == 512; }
+// z: 8..64 = rows per marching segment; 8064 / 4128 = LDS tiles of 8 columns
+// x 64 rows / 4 columns x 128 rows
 bool fused_z_seg_ok(int seg) {
-  return seg == 8 || seg == 16 || seg == 24 || seg == 32 || seg == 48 || seg == 64;
+  return seg == 8 || seg == 16 || seg == 24 || seg == 32 || seg == 48 || seg == 64 ||
+         seg == 8064 || seg == 4128;
 }
 
 int launch_dir_x(const DirArgs& a, Grid g, int nt, hipStream_t st, int members) {
@@ -591,6 +785,8 @@ int launch_dir_z(const DirArgs& a, Grid g, int seg, hipStream_t st, int members)
     case 32: return launch_z<32>(a, g, st, members);
     case 48: return launch_z<48>(a, g, st, members);
     case 64: return launch_z<64>(a, g, st, members);
+    case 8064: return launch_zt<8, 512>(a, g, st, members);
+    case 4128: return launch_zt<4, 512>(a, g, st, members);
     default: return PMW_EINVAL;
   }
 }
@@ -618,8 +814,15 @@ int plan_members(const pmw_member* mem, int n, int x_nt, int z_seg, MemberDev* d
     if (m.state == m.state_tmp || m.state == m.state_work || m.state_tmp == m.state_work)
       return PMW_EALIAS;
     const int xt = (m.nx + ens_x_cells(x_nt) - 1) / ens_x_cells(x_nt);
-    const int zc = (m.nx + 4 + 63) / 64;
-    const int zsg = (m.nz + z_seg - 1) / z_seg;
+    int zc, zsg;
+    if (z_seg >= 1000) {  // LDS tiles: CW columns x (512 / CW) rows
+      const int cw = z_seg / 1000, rows = 512 / cw - 12;
+      zc = (m.nx + cw - 1) / cw;
+      zsg = (m.nz + rows - 1) / rows;
+    } else {
+      zc = (m.nx + 4 + 63) / 64;
+      zsg = (m.nz + z_seg - 1) / z_seg;
+    }
     if (dev) {
       MemberDev& d = dev[q];
       d.state = m.state;
@@ -699,6 +902,16 @@ int launch_ens_z(const MemberDev* d, int n, int pass, int total, hipStream_t st)
   return PMW_OK;
 }
 
+template <int CW, int NT>
+int launch_ens_zt(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
+  int grid, chunk, per;
+  int rc = persistent_grid(k_ens_zt<CW, NT>, NT, total, grid, chunk, per);
+  if (rc) return rc;
+  k_ens_zt<CW, NT><<<grid, NT, 0, st>>>(d, n, pass, total, chunk, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
 int launch_ens(int dir, int x_nt, int z_seg, const MemberDev* d, int n, int pass,
                const EnsPlan& plan, hipStream_t st) {
   if (dir == 0) {
@@ -712,6 +925,8 @@ int launch_ens(int dir, int x_nt, int z_seg, const MemberDev* d, int n, int pass
     case 24: return launch_ens_z<24>(d, n, pass, plan.z_total, st);
     case 32: return launch_ens_z<32>(d, n, pass, plan.z_total, st);
     case 48: return launch_ens_z<48>(d, n, pass, plan.z_total, st);
+    case 8064: return launch_ens_zt<8, 512>(d, n, pass, plan.z_total, st);
+    case 4128: return launch_ens_zt<4, 512>(d, n, pass, plan.z_total, st);
     default: return launch_ens_z<64>(d, n, pass, plan.z_total, st);
   }
 }
