@@ -286,14 +286,16 @@ int pmw_tape_forward(double* tape, long long tape_len, int members,
 /* Reverse sweep over a tape written by pmw_tape_forward with store_stages:
  * on entry lam (one slot) is dL/d(states after step nsteps-1), on return
  * dL/d(start states) (interiors). Each step reverses its second direction,
- * then its first; each direction is three pmw_adjoint_stage launches for
- * all members. workspace: 2 slots. No reference counterpart. */
+ * then its first; each direction is one launch for all members that runs
+ * the three stage adjoints (the arithmetic of pmw_adjoint_stage) in LDS.
+ * workspace: 1 slot. tuning: NULL or {x block (256 | 384), z tile (4256 |
+ * 4384 | 2256 = columns*1000 + threads)}. No reference counterpart. */
 int pmw_tape_adjoint(const double* tape, long long tape_len, int members, double* lam,
                      double* workspace, long long workspace_len, const double* hy_dens_cell,
                      const double* hy_dens_theta_cell, const double* hy_dens_int,
                      const double* hy_dens_theta_int, int nx, int nz, double dx, double dz,
                      double dt, double hv_coeff_x, double hv_coeff_z, int reverse_direction,
-                     int nsteps, void* stream);
+                     int nsteps, const int* tuning, void* stream);
 
 /* ---- reductions -------------------------------------------------------- */
 

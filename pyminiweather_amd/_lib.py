@@ -61,7 +61,7 @@ SIGNATURES = {
     "pmw_tape_forward": (_I, [_P, _LL, _I, _P, _P, _P, _P, _P, _P, _I, _I, _D, _D, _D, _D, _D,
                               _I, _I, _I, _P, _P]),
     "pmw_tape_adjoint": (_I, [_P, _LL, _I, _P, _P, _LL, _P, _P, _P, _P, _I, _I, _D, _D, _D, _D,
-                              _D, _I, _I, _P]),
+                              _D, _I, _I, _IP, _P]),
     "pmw_stats_workspace_size": (_LL, [_I, _I]),
     "pmw_compute_stats": (_I, [_P, _P, _P, _I, _I, _D, _D, _P, _LL, _P, _P]),
     "pmw_init": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P, _I, _P]),

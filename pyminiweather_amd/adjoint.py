@@ -7,7 +7,8 @@ of the states every ``checkpoint_every`` steps (default ceil(sqrt(nsteps))).
 The backward sweep takes the segments in reverse: each is recomputed from
 its checkpoint into a tape that holds every stage state of the segment
 (``pmw_tape_forward``), then swept backwards with the hand-written adjoint
-kernels (``pmw_tape_adjoint``, six launches per step for all members).
+kernels (``pmw_tape_adjoint``: one launch per direction, all three RK
+stages, for all members).
 Memory is O(sqrt(N)) states plus one segment's tape (6 states per step);
 the recompute is one extra forward run.
 
@@ -22,6 +23,7 @@ L = 1/2 * ||(rho*theta)'(T) - target||^2 over the interior.
 
 No reference counterpart: the reference only steps forward.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -32,6 +34,8 @@ from . import solve as S
 from ._lib import ptr, stream_of
 
 RHOT = 3
+# adjoint kernels: x block threads, z tile (columns * 1000 + threads)
+ADJ_DEFAULT_TUNING = (256, 4256)
 
 
 class AdjointSolver:
@@ -57,7 +61,9 @@ class AdjointSolver:
         dev = fields.device
         n = min(self.every, self.nsteps)
         self._tape = torch.zeros((6 * n + 1,) + self.bshape, dtype=torch.float64, device=dev)
-        self._ws = torch.zeros((2,) + self.bshape, dtype=torch.float64, device=dev)
+        self._ws = torch.zeros((1,) + self.bshape, dtype=torch.float64, device=dev)
+        self.adjoint_tuning = ADJ_DEFAULT_TUNING
+        self._adj_tune = (ctypes.c_int * 2)(*self.adjoint_tuning)
         self._hvx = S.hv_coeff(self.p["dx"], self.p["dt"])
         self._hvz = S.hv_coeff(self.p["dz"], self.p["dt"])
 
@@ -71,7 +77,7 @@ class AdjointSolver:
         toggle gives in a fresh run). Returns the end-state slot (a view)."""
         f = self.f
         self._tape[0].copy_(start)
-        tune = (_lib.ctypes.c_int * 3)(*self.tuning)
+        tune = (ctypes.c_int * 3)(*self.tuning)
         _lib.call("pmw_tape_forward", ptr(self._tape), self._tape.numel(), self.members,
                   ptr(f.hy_dens_cell), ptr(f.hy_dens_theta_cell), ptr(f.hy_dens_int),
                   ptr(f.hy_dens_theta_int), ptr(f.hy_pressure_int),
@@ -108,7 +114,7 @@ class AdjointSolver:
                       ptr(f.hy_dens_theta_cell), ptr(f.hy_dens_int),
                       ptr(f.hy_dens_theta_int), f.nx, f.nz, float(self.p["dx"]),
                       float(self.p["dz"]), float(self.p["dt"]), self._hvx, self._hvz,
-                      start % 2, int(n), stream_of(f.device))
+                      start % 2, int(n), self._adj_tune, stream_of(f.device))
         return lam.reshape(lam_final.shape)
 
     # -- the theta' assimilation problem ------------------------------------

@@ -29,7 +29,8 @@
 //   6k: S_k, 6k+1: S1 of the first direction, 6k+2: its S2, 6k+3: the state
 //   after the first direction, 6k+4 / 6k+5: S1 / S2 of the second direction,
 // with S_{k+1} in slot 6k+6; pmw_tape_adjoint sweeps the steps backwards
-// from it, six adjoint launches per step.
+// from it, one launch per direction (all three adjoint stages) for all
+// members.
 #include "pmw_common.h"
 #include "pmw_fused.h"
 
@@ -296,6 +297,190 @@ int adj_stage(int dir, const double* f, const double* lam, double scale, const d
   return PMW_OK;
 }
 
+// ---- one RK3 direction in one launch --------------------------------------
+// A block holds a line tile: x = one row, NT cells along it (CW = 1); z = CW
+// columns x NR = NT/CW rows (slot t = j*CW + col). The three adjoint stages
+// run in LDS on a tile with a 7-cell halo (each stage widens the dependency
+// by 2 cells, plus 1 for the face before): lamS2 = dt T'(S2)^T lam3 on tile
+// slots 3..NR-4, lamS1 on 5..NR-6, lamS on 7..NR-8 (the NR-14 output cells).
+// Per cell and direction it reads S, S1, S2 and lam3 once and writes lamS.
+struct AdjDirArgs {
+  const double* s[3];  // forcing states of RK stages 1..3: S, S1, S2
+  const double* lam;   // cotangent of the direction's result
+  double* out;         // cotangent of its start state (must not alias lam)
+  const double* hy_cell;
+  const double* hyt_cell;
+  const double* hyr_int;
+  const double* hyt_int;
+  double dts[3];       // dt/3, dt/2, dt
+  double d, id, hv;
+  long long mstride;
+};
+
+template <int NT>
+struct AdjDirLds {
+  double st[kNv][NT];  // forcing state of the stage being reversed
+  double fw[8][NT];    // face cotangents of (vals, d3)
+  double la[kNv][NT];  // lam3, then lamS1
+  double lb[kNv][NT];  // lamS2
+};
+
+template <int DIR, int CW, int NT>
+__global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline, int nreal) {
+  constexpr int NR = NT / CW, H = 7, C = NR - 2 * H, S = DIR == 0 ? 1 : CW;
+  __shared__ AdjDirLds<NT> sm;
+  if ((int)blockIdx.x >= nreal) return;  // whole block, before any barrier
+  const long long off = (long long)blockIdx.y * a.mstride;
+  const int nx = g.nx, nz = g.nz, t = threadIdx.x;
+  const int j = t / CW, col = t % CW;
+  // line and position along it
+  int prow = 0, pcol = 0, p;
+  bool inside, wall = false;
+  int pe = 0;  // z wall slots: padded edge row
+  if (DIR == 0) {
+    const int k = blockIdx.x / nline;
+    p = (blockIdx.x % nline) * C - H + j;
+    prow = k + kHs;
+    pcol = wrapc(p, nx) + kHs;
+    inside = true;
+  } else {
+    const int i = (blockIdx.x % nline) * CW + col;
+    p = (blockIdx.x / nline) * C - H + j;
+    pcol = i + kHs;
+    const bool colok = i < nx;
+    inside = colok && p >= 0 && p < nz;
+    wall = colok && (p == -2 || p == -1 || p == nz || p == nz + 1);
+    pe = p < 0 ? kHs : nz + kHs - 1;
+    prow = p + kHs;
+  }
+  const bool own = inside && j >= H && j < H + C && (DIR == 0 ? p < nx : true);
+  // lam3 (zero outside the domain), lamS2 cleared
+  double l3[4], l2[4], l1[4];
+#pragma unroll
+  for (int v = 0; v < kNv; ++v) {
+    l3[v] = inside ? a.lam[off + g.at(v, prow, pcol)] : 0.0;
+    sm.la[v][t] = l3[v];
+    sm.lb[v][t] = 0.0;
+  }
+  auto load_state = [&](const double* src) {
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) {
+      double q = 0.0;
+      if (inside)
+        q = src[off + g.at(v, prow, pcol)];
+      else if (wall)
+        q = wall_halo(v, src[off + g.at(v, pe, pcol)], a.hy_cell[pe], a.hy_cell[prow]);
+      sm.st[v][t] = q;
+    }
+  };
+  load_state(a.s[2]);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {  // m = 0: stage 3 (forcing S2), 1: stage 2, 2: stage 1
+    const int stage = 2 - m;
+    double (*lin)[NT] = m == 1 ? sm.lb : sm.la;
+    const double sc = a.dts[stage];
+    const double sd = sc * a.id;
+    // faces at slots 2+2m .. NR-2-2m (z: faces outside 0..nz are zero)
+    const bool zout = DIR == 1 && (p < 0 || p > nz);
+    if (zout) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) sm.fw[q][t] = 0.0;
+    } else if (j >= 2 + 2 * m && j <= NR - 2 - 2 * m && (DIR == 0 || col + (blockIdx.x % nline) * CW < nx)) {
+      double val[4], d3[4], lf[4], lv[4], ld[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v)
+        recon4(sm.st[v][t - 2 * S], sm.st[v][t - S], sm.st[v][t], sm.st[v][t + S], val[v], d3[v]);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) lf[v] = sd * (lin[v][t] - lin[v][t - S]);
+      if (DIR == 0)
+        adj_flux_x(val, a.hy_cell[prow], a.hyt_cell[prow], a.hv, lf, lv, ld);
+      else
+        adj_flux_z(val, a.hyr_int[p], a.hyt_int[p], a.hv, p == 0 || p == nz, lf, lv, ld);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) {
+        sm.fw[v][t] = lv[v];
+        sm.fw[4 + v][t] = ld[v];
+      }
+    }
+    __syncthreads();
+    // gravity needs lin[w] at this slot: read before lin may be overwritten
+    const double lw = lin[2][t];
+    // cells at slots 3+2m .. NR-4-2m
+    double l[4] = {0.0, 0.0, 0.0, 0.0};
+    const bool cell = inside && j >= 3 + 2 * m && j <= NR - 4 - 2 * m;
+    if (cell) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int sl = t + (2 - q) * S;
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) l[v] += kW4[q] * sm.fw[v][sl] + kD3[q] * sm.fw[4 + v][sl];
+      }
+      if (DIR == 1) {
+        if (p == 0 || p == nz - 1) {
+          // the wall halo rows of this edge row: positions -2, -1 or nz, nz+1
+          const int r0w = p == 0 ? -2 : nz;
+          for (int r = r0w; r < r0w + 2; ++r) {
+            double h[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int f = r + 2 - q;
+              if (f < 0 || f > nz) continue;
+              const int sl = t + (f - p) * S;
+#pragma unroll
+              for (int v = 0; v < kNv; ++v) h[v] += kW4[q] * sm.fw[v][sl] + kD3[q] * sm.fw[4 + v][sl];
+            }
+            l[0] += h[0];
+            l[1] += h[1] * a.hy_cell[r + kHs] / a.hy_cell[prow];
+            l[3] += h[3];
+          }
+        }
+        l[0] += -kGrav * (sc * lw);
+      }
+    }
+    if (m < 2) {
+      // next forcing state; st is free once the faces are done
+      load_state(a.s[stage - 1]);
+      if (cell) {
+        double (*lout)[NT] = m == 0 ? sm.lb : sm.la;
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) {
+          lout[v][t] = l[v];
+          if (m == 0) l2[v] = l[v]; else l1[v] = l[v];
+        }
+      }
+      __syncthreads();
+    } else if (own) {
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) {
+        double r = l[v];
+        r += l3[v];
+        r += l2[v];
+        r += l1[v];
+        a.out[off + g.at(v, prow, pcol)] = r;
+      }
+    }
+  }
+}
+
+template <int DIR, int CW, int NT>
+int launch_adj_dir(const AdjDirArgs& a, Grid g, int members, hipStream_t st) {
+  constexpr int C = NT / CW - 14;
+  int nline;
+  long long nreal;
+  if (DIR == 0) {
+    nline = (g.nx + C - 1) / C;
+    nreal = (long long)nline * g.nz;
+  } else {
+    nline = (g.nx + CW - 1) / CW;
+    nreal = (long long)nline * ((g.nz + C - 1) / C);
+  }
+  if (nreal > (1LL << 31) - 1) return PMW_EINVAL;
+  k_adj_dir<DIR, CW, NT><<<dim3((unsigned)nreal, members), NT, 0, st>>>(a, g, nline, (int)nreal);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
 bool grid_ok(int nx, int nz) {
   // blocks per launch stay below 2^31 and rows index as int
   return nx >= 2 && nz >= 2 && (long long)(nx + 4) * (nz + 4) * 4 < (1LL << 40);
@@ -387,41 +572,56 @@ int pmw_tape_adjoint(const double* tape, long long tape_len, int members, double
                      const double* hy_dens_theta_cell, const double* hy_dens_int,
                      const double* hy_dens_theta_int, int nx, int nz, double dx, double dz,
                      double dt, double hv_coeff_x, double hv_coeff_z, int reverse_direction,
-                     int nsteps, void* stream) {
+                     int nsteps, const int* tuning, void* stream) {
   const long long slot = pmw_tape_slot_len(nx, nz, members);
   if (slot == 0 || !tape || !lam || !workspace || !hy_dens_cell || !hy_dens_theta_cell ||
       !hy_dens_int || !hy_dens_theta_int || nsteps < 0 || nsteps > 100000)
     return PMW_EINVAL;
-  if (tape_len < (6LL * nsteps + 1) * slot || workspace_len < 2 * slot) return PMW_ESPACE;
+  const int x_nt = tuning ? tuning[0] : 256;
+  const int z_tile = tuning ? tuning[1] : 4256;
+  if ((x_nt != 256 && x_nt != 384) || (z_tile != 4256 && z_tile != 4384 && z_tile != 2256))
+    return PMW_EINVAL;
+  if (tape_len < (6LL * nsteps + 1) * slot || workspace_len < slot) return PMW_ESPACE;
   const double* tape_end = tape + (6LL * nsteps + 1) * slot;
   if ((lam >= tape && lam < tape_end) || (workspace >= tape && workspace < tape_end) ||
-      (lam >= workspace && lam < workspace + 2 * slot))
+      (lam >= workspace && lam < workspace + slot) || (workspace >= lam && workspace < lam + slot))
     return PMW_EALIAS;
   Grid g(nx, nz);
   hipStream_t st = (hipStream_t)stream;
-  double* l1 = workspace;
-  double* l2 = workspace + slot;
-  const double dts[3] = {dt / 3, dt / 2, dt / 1};
-  const long long ms = state_len(nx, nz);
+  AdjDirArgs a;
+  a.hy_cell = hy_dens_cell;
+  a.hyt_cell = hy_dens_theta_cell;
+  a.hyr_int = hy_dens_int;
+  a.hyt_int = hy_dens_theta_int;
+  a.dts[0] = dt / 3;
+  a.dts[1] = dt / 2;
+  a.dts[2] = dt / 1;
+  a.mstride = state_len(nx, nz);
+  // the cotangent ping-pongs lam -> workspace -> lam within each step
+  double* bufs[2] = {lam, workspace};
   for (int k = nsteps - 1; k >= 0; --k) {
     const int rev = (reverse_direction != 0) ^ (k & 1);
-    const int first = rev ? 0 : 1;
+    const int first = rev ? 0 : 1;  // 0 = x, 1 = z
     for (int pass = 1; pass >= 0; --pass) {
       const int dir = pass == 0 ? first : 1 - first;
       const double* s0 = tape + (6LL * k + 3 * pass) * slot;
-      const double* s1 = s0 + slot;
-      const double* s2 = s0 + 2 * slot;
-      const double d = dir == 0 ? dx : dz;
-      const double hv = dir == 0 ? hv_coeff_x : hv_coeff_z;
-      auto adj = [&](const double* f, const double* lin, double sc, const double* a1,
-                     const double* a2, const double* a3, double* lout) {
-        return adj_stage(dir, f, lin, sc, a1, a2, a3, lout, hy_dens_cell, hy_dens_theta_cell,
-                         hy_dens_int, hy_dens_theta_int, g, d, hv, members, ms, st);
-      };
+      a.s[0] = s0;
+      a.s[1] = s0 + slot;
+      a.s[2] = s0 + 2 * slot;
+      a.lam = bufs[pass == 1 ? 0 : 1];
+      a.out = bufs[pass == 1 ? 1 : 0];
+      a.d = dir == 0 ? dx : dz;
+      a.id = 1.0 / a.d;
+      a.hv = dir == 0 ? hv_coeff_x : hv_coeff_z;
       int rc;
-      if ((rc = adj(s2, lam, dts[2], nullptr, nullptr, nullptr, l2))) return rc;
-      if ((rc = adj(s1, l2, dts[1], nullptr, nullptr, nullptr, l1))) return rc;
-      if ((rc = adj(s0, l1, dts[0], lam, l2, l1, lam))) return rc;
+      if (dir == 0)
+        rc = x_nt == 256 ? launch_adj_dir<0, 1, 256>(a, g, members, st)
+                         : launch_adj_dir<0, 1, 384>(a, g, members, st);
+      else
+        rc = z_tile == 4256   ? launch_adj_dir<1, 4, 256>(a, g, members, st)
+             : z_tile == 4384 ? launch_adj_dir<1, 4, 384>(a, g, members, st)
+                              : launch_adj_dir<1, 2, 256>(a, g, members, st);
+      if (rc) return rc;
     }
   }
   return PMW_OK;
