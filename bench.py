@@ -46,13 +46,14 @@ def traffic_from_profile(kernel_prefix):
     """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
     summary (FETCH_SIZE and WRITE_SIZE scaled by a plain-copy calibration),
     or None if the summary has no line for it."""
+    import re
+    pat = re.compile(r"^traffic_bytes_per_launch (.+) (\d+)\s+\(read")
     try:
         with open(PROFILE_SUMMARY) as fh:
             for line in fh:
-                parts = line.split()
-                if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch" \
-                        and parts[1].startswith(kernel_prefix):
-                    return float(parts[2])
+                m = pat.match(line)
+                if m and m.group(1).startswith(kernel_prefix):
+                    return float(m.group(2))
     except OSError:
         pass
     return None
@@ -141,7 +142,7 @@ def bench_ensemble(nsteps, loop_steps, autotune):
             "speedup_vs_loop": rate / loop_rate,
             "step_roofline_frac": round(rate * BYTES_PER_CELL_STEP / 1e9 / HBM_PEAK_GBS, 4),
             "max_rel_mass_drift": drift,
-            "launch": "one persistent launch per direction and step, 2-step HIP graph",
+            "launch": "one launch per direction and step over the flattened tile list of all members, 2-step HIP graph",
             "tuning": list(ens.tuning)}
 
 

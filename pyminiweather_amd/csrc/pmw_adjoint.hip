@@ -536,12 +536,12 @@ int pmw_tape_forward(double* tape, long long tape_len, int members,
   Grid g(nx, nz);
   hipStream_t st = (hipStream_t)stream;
   DirArgs a;
-  a.hy_cell = hy_dens_cell;
-  a.hyt_cell = hy_dens_theta_cell;
-  a.hyr_int = hy_dens_int;
-  a.hyt_int = hy_dens_theta_int;
-  a.hyp_int = hy_pressure_int;
-  a.src_w = src_w;
+  a.hy_cell = gptr(hy_dens_cell);
+  a.hyt_cell = gptr(hy_dens_theta_cell);
+  a.hyr_int = gptr(hy_dens_int);
+  a.hyt_int = gptr(hy_dens_theta_int);
+  a.hyp_int = gptr(hy_pressure_int);
+  a.src_w = gptr(src_w);
   a.dt1 = dt / 3;
   a.dt2 = dt / 2;
   a.dt3 = dt / 1;
@@ -552,11 +552,11 @@ int pmw_tape_forward(double* tape, long long tape_len, int members,
     for (int pass = 0; pass < 2; ++pass) {
       const int dir = pass == 0 ? first : 1 - first;
       double* base = tape + (6LL * k + 3 * pass) * slot;
-      a.in = base;
-      a.st = base;  // halo refresh of the start state (never read by this launch)
-      a.s1 = store_stages ? base + slot : nullptr;
-      a.tmp = base + 2 * slot;
-      a.out = base + 3 * slot;
+      a.in = gptr(base);
+      a.st = gptr(base);  // halo refresh of the start state (never read by this launch)
+      a.s1 = store_stages ? gptr(base + slot) : nullptr;
+      a.tmp = gptr(base + 2 * slot);
+      a.out = gptr(base + 3 * slot);
       a.d = dir == 0 ? dx : dz;
       a.hv = dir == 0 ? hv_coeff_x : hv_coeff_z;
       const int rc = dir == 0 ? launch_dir_x(a, g, x_nt, st, members)

@@ -15,6 +15,21 @@
 
 namespace pmw {
 
+// Device-global pointers. A pointer read from memory (a descriptor array)
+// is generic to the compiler, which then emits flat_* accesses that also
+// count against the LDS wait counter; typed in the global address space it
+// emits global_* instead. Host code sees plain pointers (same layout).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PMW_GLOBAL __attribute__((address_space(1)))
+#else
+#define PMW_GLOBAL
+#endif
+using gdouble = PMW_GLOBAL double;
+template <typename T>
+__host__ __device__ inline PMW_GLOBAL T* gptr(T* p) {
+  return (PMW_GLOBAL T*)p;
+}
+
 // pyminiweather/data/constants.py:13-27 (rd is 287 in a float Enum -> 287.0)
 constexpr double kHvBeta = 0.05;
 constexpr double kP0 = 1.0e5;

@@ -7,20 +7,20 @@ namespace pmw {
 
 // Arguments of one fused direction (three RK stages) of one grid.
 struct DirArgs {
-  const double* in;   // start state of the direction (interior read; RK init)
-  double* st;         // receives the stage-1 halo refresh (reference `state`)
-  double* out;        // result interior; must not alias `in`
-  double* tmp;        // stage-2 interior + stage-3 halo refresh (`state_tmp`)
-  const double* hy_cell;   // hy_dens_cell (x flux rows, z wall halos)
-  const double* hyt_cell;  // hy_dens_theta_cell (x flux rows)
-  const double* hyr_int;   // hy_dens_int (z faces)
-  const double* hyt_int;   // hy_dens_theta_int (z faces)
-  const double* hyp_int;   // hy_pressure_int (z faces)
-  const double* src_w;     // (nz, nx) w source or nullptr
+  const gdouble* in;   // start state of the direction (interior read; RK init)
+  gdouble* st;         // receives the stage-1 halo refresh (reference `state`)
+  gdouble* out;        // result interior; must not alias `in`
+  gdouble* tmp;        // stage-2 interior + stage-3 halo refresh (`state_tmp`)
+  const gdouble* hy_cell;   // hy_dens_cell (x flux rows, z wall halos)
+  const gdouble* hyt_cell;  // hy_dens_theta_cell (x flux rows)
+  const gdouble* hyr_int;   // hy_dens_int (z faces)
+  const gdouble* hyt_int;   // hy_dens_theta_int (z faces)
+  const gdouble* hyp_int;   // hy_pressure_int (z faces)
+  const gdouble* src_w;     // (nz, nx) w source or nullptr
   double d;                // dx or dz
   double dt1, dt2, dt3;    // dt/3, dt/2, dt
   double hv;               // -hv_beta * d / (16 * dt_full)
-  double* s1 = nullptr;    // optional: receives the stage-1 interior
+  gdouble* s1 = nullptr;   // optional: receives the stage-1 interior
   long long mstride = 0;   // batched launches: elements between members of
                            // in / st / out / tmp / s1 (member = blockIdx.y)
 };

@@ -228,7 +228,7 @@ __device__ __forceinline__ bool z_stage_step(double w[4][4], double F[4], const 
                                              const double cur[4], double dts, double res[4]) {
   constexpr int S0 = (PH + 1) & 3, S1 = (PH + 2) & 3, S2 = (PH + 3) & 3, S3 = PH & 3;
   const int nz = g.nz;
-  const double* hy = a.hy_cell;
+  const gdouble* hy = a.hy_cell;
   // the new row is formed in nv and stored once, so that every window slot
   // is written through a static index (no merged private-array stores)
   if (r >= lo - 2 && r <= hi + 1 && r < nz + 2) {
@@ -298,7 +298,7 @@ __device__ __forceinline__ bool z_stage_step(double w[4][4], double F[4], const 
   return done;
 }
 
-__device__ __forceinline__ void store_row(double* buf, const Grid& g, int prow, int pc,
+__device__ __forceinline__ void store_row(gdouble* buf, const Grid& g, int prow, int pc,
                                           const double q[4]) {
 #pragma unroll
   for (int v = 0; v < kNv; ++v) buf[g.at(v, prow, pc)] = q[v];
@@ -312,7 +312,7 @@ __device__ __forceinline__ void z_block_step(double w[4][4], double F[4], const 
                                              int k1, int pc, int i, int lane, bool active,
                                              const double cur[4], double dts) {
   const int nz = g.nz;
-  const double* hy = a.hy_cell;
+  const gdouble* hy = a.hy_cell;
   const int r = j - 3 * stage;  // input row of this stage at step j
   if (active) {
     // rows of S1 / S2 come from the previous stage's ring (written one step ago)
@@ -421,14 +421,14 @@ __device__ __forceinline__ void dir_z_block(const DirArgs& a, const Grid& g, int
   const int stage = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = colgrp * 64 + lane;  // interior column
   const int k0 = seg * L, k1 = min(k0 + L, nz);
-  const double* hy = a.hy_cell;
+  const gdouble* hy = a.hy_cell;
   if (stage == 0 && i >= nx && i < nx + 4) {
     // corner columns of the halo rows (padded 0, 1, nx+2, nx+3) of st and tmp
     const int q = i - nx;
     const int cc = q < 2 ? q : nx + q;
     const int lo = kHs, hi = nz + kHs - 1;
     for (int pass = 0; pass < 2; ++pass) {
-      double* buf = pass == 0 ? a.st : a.tmp;
+      gdouble* buf = pass == 0 ? a.st : a.tmp;
       if (k0 == 0) {
 #pragma unroll
         for (int v = 0; v < kNv; ++v) {
@@ -493,7 +493,7 @@ __device__ __forceinline__ void dir_zt_tile(const DirArgs& a, const Grid& g, int
   const int k = r0 - H + j;   // logical row of this slot
   const bool colok = i < nx;
   const int pc = i + kHs;
-  const double* hy = a.hy_cell;
+  const gdouble* hy = a.hy_cell;
   const int plo = kHs, phi = nz + kHs - 1;  // padded edge rows
   const bool wlo = k == -2 || k == -1, whi = k == nz || k == nz + 1;
   const bool own = colok && k >= r0 && k < r0 + C && k < nz;
@@ -521,7 +521,7 @@ __device__ __forceinline__ void dir_zt_tile(const DirArgs& a, const Grid& g, int
   }
   if (corners && t < 8) {
     // corner columns (padded 0, 1, nx+2, nx+3) of the halo rows of st, tmp
-    double* buf = t < 4 ? a.st : a.tmp;
+    gdouble* buf = t < 4 ? a.st : a.tmp;
     const int qq = t & 3;
     const int cc = qq < 2 ? qq : nx + qq;
 #pragma unroll
@@ -568,7 +568,7 @@ __device__ __forceinline__ void dir_zt_tile(const DirArgs& a, const Grid& g, int
         for (int v = 0; v < kNv; ++v) s1[v][t] = q[v];
       }
       if (own) {
-        double* dst = stage == 0 ? a.s1 : stage == 1 ? a.tmp : a.out;
+        gdouble* dst = stage == 0 ? a.s1 : stage == 1 ? a.tmp : a.out;
         if (dst) {
 #pragma unroll
           for (int v = 0; v < kNv; ++v) dst[g.at(v, k + kHs, pc)] = q[v];
@@ -626,16 +626,16 @@ struct MemberDev {
 
 __device__ __forceinline__ DirArgs member_args(const MemberDev& m, int dir, int pass) {
   DirArgs a;
-  a.in = pass == 0 ? m.state : m.work;
-  a.out = pass == 0 ? m.work : m.state;
-  a.st = m.state;
-  a.tmp = m.tmp;
-  a.hy_cell = m.hyc;
-  a.hyt_cell = m.hytc;
-  a.hyr_int = m.hyri;
-  a.hyt_int = m.hyti;
-  a.hyp_int = m.hypi;
-  a.src_w = m.src;
+  a.in = gptr(pass == 0 ? m.state : m.work);
+  a.out = gptr(pass == 0 ? m.work : m.state);
+  a.st = gptr(m.state);
+  a.tmp = gptr(m.tmp);
+  a.hy_cell = gptr(m.hyc);
+  a.hyt_cell = gptr(m.hytc);
+  a.hyr_int = gptr(m.hyri);
+  a.hyt_int = gptr(m.hyti);
+  a.hyp_int = gptr(m.hypi);
+  a.src_w = m.src ? gptr(m.src) : nullptr;
   a.d = dir == 0 ? m.dx : m.dz;
   a.dt1 = m.dt1;
   a.dt2 = m.dt2;
@@ -644,80 +644,62 @@ __device__ __forceinline__ DirArgs member_args(const MemberDev& m, int dir, int 
   return a;
 }
 
-// Member owning work item `item` (starts are ascending); `mi` is a hint.
+// Member owning work item `item`: binary search over the ascending start
+// offsets (block-uniform, scalar loads).
 template <int DIR>
-__device__ __forceinline__ int member_of(const MemberDev* m, int n, int item, int mi) {
+__device__ __forceinline__ int member_of(const MemberDev* m, int n, int item) {
   auto start = [&](int q) { return DIR == 0 ? m[q].x_start : m[q].z_start; };
-  if (start(mi) > item) {  // binary search from scratch
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (start(mid) <= item) lo = mid; else hi = mid - 1;
-    }
-    return lo;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start(mid) <= item) lo = mid; else hi = mid - 1;
   }
-  while (mi + 1 < n && start(mi + 1) <= item) ++mi;
-  return mi;
+  return lo;
 }
 
-// Persistent ensemble kernels: block b (XCD-remapped) takes the contiguous
-// chunk of work items [lb*chunk, (lb+1)*chunk), so neighbouring tiles of a
-// member share an XCD's L2. Every loop bound is block-uniform.
+// Ensemble kernels: one launch covers the flattened work list of every
+// member (tiles of member q are items x_start[q] .. ); block b takes item
+// xcd_logical(b), so neighbouring tiles of a member share an XCD's L2, and
+// finds its member by a scalar binary search over the start offsets. (A
+// persistent form, each block looping over a chunk of items, kept ~70 more
+// VGPRs live and halved the occupancy.)
 template <int NT>
 __global__ __launch_bounds__(NT) void k_ens_x(const MemberDev* __restrict__ m, int n, int pass,
-                                              int total, int chunk, int per_xcd) {
+                                              int total, int per_xcd) {
   constexpr int C = NT - 12;
   __shared__ XTileLds<NT> sm;
-  const int lb = xcd_logical(blockIdx.x, per_xcd);
-  const int first = lb * chunk;
-  const int last = min(total, first + chunk);
-  int mi = 0;
-  for (int item = first; item < last; ++item) {
-    mi = member_of<0>(m, n, item, mi);
-    const MemberDev& md = m[mi];
-    const DirArgs a = member_args(md, 0, pass);
-    const int local = item - md.x_start;
-    dir_x_tile<NT>(a, Grid(md.nx, md.nz), local / md.x_ntiles, (local % md.x_ntiles) * C, sm);
-    __syncthreads();  // the next tile rewrites the LDS this one last read
-  }
+  const int item = xcd_logical(blockIdx.x, per_xcd);
+  if (item >= total) return;  // whole block, before any barrier
+  const MemberDev& md = m[member_of<0>(m, n, item)];
+  const DirArgs a = member_args(md, 0, pass);
+  const int local = item - md.x_start;
+  dir_x_tile<NT>(a, Grid(md.nx, md.nz), local / md.x_ntiles, (local % md.x_ntiles) * C, sm);
 }
 
 template <int L>
 __global__ __launch_bounds__(192) void k_ens_z(const MemberDev* __restrict__ m, int n, int pass,
-                                               int total, int chunk, int per_xcd) {
+                                               int total, int per_xcd) {
   __shared__ double ring[2][2][kNv][64];
-  const int lb = xcd_logical(blockIdx.x, per_xcd);
-  const int first = lb * chunk;
-  const int last = min(total, first + chunk);
-  int mi = 0;
-  for (int item = first; item < last; ++item) {
-    mi = member_of<1>(m, n, item, mi);
-    const MemberDev& md = m[mi];
-    const DirArgs a = member_args(md, 1, pass);
-    const int local = item - md.z_start;
-    dir_z_block<L>(a, Grid(md.nx, md.nz), local % md.z_ncol, local / md.z_ncol, ring);
-    // dir_z_block ends on a barrier, so the ring is free for the next item
-  }
+  const int item = xcd_logical(blockIdx.x, per_xcd);
+  if (item >= total) return;  // whole block, before any barrier
+  const MemberDev& md = m[member_of<1>(m, n, item)];
+  const DirArgs a = member_args(md, 1, pass);
+  const int local = item - md.z_start;
+  dir_z_block<L>(a, Grid(md.nx, md.nz), local % md.z_ncol, local / md.z_ncol, ring);
 }
 
 template <int CW, int NT>
 __global__ __launch_bounds__(NT) void k_ens_zt(const MemberDev* __restrict__ m, int n, int pass,
-                                               int total, int chunk, int per_xcd) {
+                                               int total, int per_xcd) {
   constexpr int C = NT / CW - 12;
   __shared__ XTileLds<NT> sm;
-  const int lb = xcd_logical(blockIdx.x, per_xcd);
-  const int first = lb * chunk;
-  const int last = min(total, first + chunk);
-  int mi = 0;
-  for (int item = first; item < last; ++item) {
-    mi = member_of<1>(m, n, item, mi);
-    const MemberDev& md = m[mi];
-    const DirArgs a = member_args(md, 1, pass);
-    const int local = item - md.z_start;
-    dir_zt_tile<CW, NT>(a, Grid(md.nx, md.nz), (local % md.z_ncol) * CW,
-                        (local / md.z_ncol) * C, sm, local == 0);
-    __syncthreads();  // the next tile rewrites the LDS this one last read
-  }
+  const int item = xcd_logical(blockIdx.x, per_xcd);
+  if (item >= total) return;  // whole block, before any barrier
+  const MemberDev& md = m[member_of<1>(m, n, item)];
+  const DirArgs a = member_args(md, 1, pass);
+  const int local = item - md.z_start;
+  dir_zt_tile<CW, NT>(a, Grid(md.nx, md.nz), (local % md.z_ncol) * CW,
+                      (local / md.z_ncol) * C, sm, local == 0);
 }
 
 template <int NT>
@@ -865,49 +847,28 @@ int ens_tuning(const int* t, int& x_nt, int& z_seg) {
   return PMW_OK;
 }
 
-template <typename K>
-int persistent_grid(K kernel, int threads, int total, int& grid, int& chunk, int& per) {
-  int dev = 0, ncu = 0, per_cu = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0);
-  if (e != hipSuccess) return (int)e;
-  long long want = (long long)ncu * (per_cu > 0 ? per_cu : 1);
-  if (want > total) want = total;
-  want = (want + kNumXcd - 1) / kNumXcd * kNumXcd;
-  per = (int)(want / kNumXcd);
-  grid = (int)want;
-  chunk = (int)((total + want - 1) / want);
-  return PMW_OK;
-}
+inline int ens_per_xcd(int total) { return (total + kNumXcd - 1) / kNumXcd; }
 
 template <int NT>
 int launch_ens_x(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
-  int grid, chunk, per;
-  int rc = persistent_grid(k_ens_x<NT>, NT, total, grid, chunk, per);
-  if (rc) return rc;
-  k_ens_x<NT><<<grid, NT, 0, st>>>(d, n, pass, total, chunk, per);
+  const int per = ens_per_xcd(total);
+  k_ens_x<NT><<<per * kNumXcd, NT, 0, st>>>(d, n, pass, total, per);
   PMW_CHECK_LAUNCH();
   return PMW_OK;
 }
 
 template <int L>
 int launch_ens_z(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
-  int grid, chunk, per;
-  int rc = persistent_grid(k_ens_z<L>, 192, total, grid, chunk, per);
-  if (rc) return rc;
-  k_ens_z<L><<<grid, 192, 0, st>>>(d, n, pass, total, chunk, per);
+  const int per = ens_per_xcd(total);
+  k_ens_z<L><<<per * kNumXcd, 192, 0, st>>>(d, n, pass, total, per);
   PMW_CHECK_LAUNCH();
   return PMW_OK;
 }
 
 template <int CW, int NT>
 int launch_ens_zt(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
-  int grid, chunk, per;
-  int rc = persistent_grid(k_ens_zt<CW, NT>, NT, total, grid, chunk, per);
-  if (rc) return rc;
-  k_ens_zt<CW, NT><<<grid, NT, 0, st>>>(d, n, pass, total, chunk, per);
+  const int per = ens_per_xcd(total);
+  k_ens_zt<CW, NT><<<per * kNumXcd, NT, 0, st>>>(d, n, pass, total, per);
   PMW_CHECK_LAUNCH();
   return PMW_OK;
 }

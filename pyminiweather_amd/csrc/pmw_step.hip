@@ -655,16 +655,16 @@ int evolve_impl(double* state, double* state_tmp, double* state_work,
       if (fused) {
         // pass 0: S -> W, pass 1: W -> S; halos of stage 1 always into S
         DirArgs a;
-        a.in = pass == 0 ? state : state_work;
-        a.out = pass == 0 ? state_work : state;
-        a.st = state;
-        a.tmp = state_tmp;
-        a.hy_cell = hy_dens_cell;
-        a.hyt_cell = hy_dens_theta_cell;
-        a.hyr_int = hy_dens_int;
-        a.hyt_int = hy_dens_theta_int;
-        a.hyp_int = hy_pressure_int;
-        a.src_w = src_w;
+        a.in = gptr(pass == 0 ? state : state_work);
+        a.out = gptr(pass == 0 ? state_work : state);
+        a.st = gptr(state);
+        a.tmp = gptr(state_tmp);
+        a.hy_cell = gptr(hy_dens_cell);
+        a.hyt_cell = gptr(hy_dens_theta_cell);
+        a.hyr_int = gptr(hy_dens_int);
+        a.hyt_int = gptr(hy_dens_theta_int);
+        a.hyp_int = gptr(hy_pressure_int);
+        a.src_w = gptr(src_w);
         a.d = dir == 0 ? dx : dz;
         a.dt1 = dts[0];
         a.dt2 = dts[1];
