@@ -525,7 +525,7 @@ int pmw_tape_forward(double* tape, long long tape_len, int members,
   if (slot == 0 || !tape || !hy_dens_cell || !hy_dens_theta_cell || !hy_dens_int ||
       !hy_dens_theta_int || !hy_pressure_int || nsteps < 0 || nsteps > 100000)
     return PMW_EINVAL;
-  int x_nt = 256, z_seg = 16;
+  int x_nt = 512, z_seg = 4128;
   if (tuning) {
     if (tuning[0] != 0 || !fused_x_tile_ok(tuning[1]) || !fused_z_seg_ok(tuning[2]))
       return PMW_EINVAL;

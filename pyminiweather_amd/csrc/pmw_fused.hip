@@ -840,8 +840,8 @@ int plan_members(const pmw_member* mem, int n, int x_nt, int z_seg, MemberDev* d
 }
 
 int ens_tuning(const int* t, int& x_nt, int& z_seg) {
-  x_nt = t ? t[1] : 256;
-  z_seg = t ? t[2] : 16;
+  x_nt = t ? t[1] : 512;
+  z_seg = t ? t[2] : 4128;
   if (t && t[0] != 0) return PMW_EINVAL;  // ensembles run the fused kernels only
   if (!pmw::fused_x_tile_ok(x_nt) || !pmw::fused_z_seg_ok(z_seg)) return PMW_EINVAL;
   return PMW_OK;

@@ -56,7 +56,7 @@ def mixed_thermal_members(n=256, seed=0, grids=GRID_CHOICES):
 
 
 class Ensemble:
-    def __init__(self, params_list, bubbles=None, device=None, tuning=(0, 256, 16)):
+    def __init__(self, params_list, bubbles=None, device=None, tuning=(0, 512, 4128)):
         if device is None:
             device = torch.device("cuda", torch.cuda.current_device())
         self.device = device
