@@ -60,7 +60,7 @@ __device__ __forceinline__ void face_prims(const double val[4], double hyr, doub
   u = val[1] * ir;
   w = val[2] * ir;
   t = q * ir;
-  dp = (kGamma * kC0) * pow(q, kGamma - 1.0);
+  dp = (kGamma * kC0) * pow_pos(q, kGamma - 1.0);
 }
 
 // Transposed x-flux Jacobian: cotangents of (vals, d3) from lamF.

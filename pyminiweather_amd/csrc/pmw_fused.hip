@@ -594,9 +594,11 @@ __device__ __forceinline__ void dir_zt_tile(const DirArgs& a, const Grid& g, int
   }
 }
 
+// 6 waves per SIMD (the LDS limit of a 512-thread tile): left alone the
+// compiler allocates ~106 VGPRs (4 waves); asked, it fits 72 without spills.
 template <int CW, int NT>
-__global__ __launch_bounds__(NT) void k_dir_zt(DirArgs a, Grid g, int ncolgrp, int nreal,
-                                               int per_xcd) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_dir_zt(
+    DirArgs a, Grid g, int ncolgrp, int nreal, int per_xcd) {
   constexpr int C = NT / CW - 12;
   __shared__ XTileLds<NT> sm;
   const int logical = xcd_logical(blockIdx.x, per_xcd);
@@ -689,8 +691,8 @@ __global__ __launch_bounds__(192) void k_ens_z(const MemberDev* __restrict__ m, 
 }
 
 template <int CW, int NT>
-__global__ __launch_bounds__(NT) void k_ens_zt(const MemberDev* __restrict__ m, int n, int pass,
-                                               int total, int per_xcd) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_ens_zt(
+    const MemberDev* __restrict__ m, int n, int pass, int total, int per_xcd) {
   constexpr int C = NT / CW - 12;
   __shared__ XTileLds<NT> sm;
   const int item = xcd_logical(blockIdx.x, per_xcd);

@@ -115,7 +115,7 @@ __global__ void k_init_hydro(int ic, double* __restrict__ hy_dens_cell,
     if (ic == PMW_IC_GRAVITY) hydro_bvfreq(z, hr, ht); else hydro_theta(z, hr, ht);
     hy_dens_int[k] = hr;
     hy_dens_theta_int[k] = hr * ht;
-    hy_pressure_int[k] = kC0 * pow(hr * ht, kGamma);
+    hy_pressure_int[k] = kC0 * pow_pos(hr * ht, kGamma);
   }
 }
 
@@ -135,7 +135,7 @@ __global__ void k_init_source_w(double* __restrict__ src, const double* __restri
 __global__ void k_hydro_pressure(const double* __restrict__ hy_dens_theta_int,
                                  double* __restrict__ hy_pressure_int, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) hy_pressure_int[k] = kC0 * pow(hy_dens_theta_int[k], kGamma);
+  if (k < n) hy_pressure_int[k] = kC0 * pow_pos(hy_dens_theta_int[k], kGamma);
 }
 
 }  // namespace
