@@ -60,7 +60,7 @@ __device__ __forceinline__ void face_prims(const double val[4], double hyr, doub
   u = val[1] * ir;
   w = val[2] * ir;
   t = q * ir;
-  dp = (kGamma * kC0) * pow_pos(q, kGamma - 1.0);
+  dp = (kGamma * kC0) * pow_pos_fast(q, kGamma - 1.0);  // a derivative: few-ulp power
 }
 
 // Transposed x-flux Jacobian: cotangents of (vals, d3) from lamF.
@@ -325,11 +325,19 @@ struct AdjDirLds {
   double lb[kNv][NT];  // lamS2
 };
 
+// Consecutive tiles (neighbouring columns / row segments) go to the same XCD
+// so that the cache lines they share are fetched into one L2.
+__device__ __forceinline__ int adj_xcd_logical(int b, int per_xcd) {
+  return (b % 8) * per_xcd + b / 8;
+}
+
 template <int DIR, int CW, int NT>
-__global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline, int nreal) {
+__global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline, int nreal,
+                                                int per_xcd) {
   constexpr int NR = NT / CW, H = 7, C = NR - 2 * H, S = DIR == 0 ? 1 : CW;
   __shared__ AdjDirLds<NT> sm;
-  if ((int)blockIdx.x >= nreal) return;  // whole block, before any barrier
+  const int blk = adj_xcd_logical(blockIdx.x, per_xcd);
+  if (blk >= nreal) return;  // whole block, before any barrier
   const long long off = (long long)blockIdx.y * a.mstride;
   const int nx = g.nx, nz = g.nz, t = threadIdx.x;
   const int j = t / CW, col = t % CW;
@@ -338,14 +346,14 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
   bool inside, wall = false;
   int pe = 0;  // z wall slots: padded edge row
   if (DIR == 0) {
-    const int k = blockIdx.x / nline;
-    p = (blockIdx.x % nline) * C - H + j;
+    const int k = blk / nline;
+    p = (blk % nline) * C - H + j;
     prow = k + kHs;
     pcol = wrapc(p, nx) + kHs;
     inside = true;
   } else {
-    const int i = (blockIdx.x % nline) * CW + col;
-    p = (blockIdx.x / nline) * C - H + j;
+    const int i = (blk % nline) * CW + col;
+    p = (blk / nline) * C - H + j;
     pcol = i + kHs;
     const bool colok = i < nx;
     inside = colok && p >= 0 && p < nz;
@@ -386,7 +394,7 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
     if (zout) {
 #pragma unroll
       for (int q = 0; q < 8; ++q) sm.fw[q][t] = 0.0;
-    } else if (j >= 2 + 2 * m && j <= NR - 2 - 2 * m && (DIR == 0 || col + (blockIdx.x % nline) * CW < nx)) {
+    } else if (j >= 2 + 2 * m && j <= NR - 2 - 2 * m && (DIR == 0 || col + (blk % nline) * CW < nx)) {
       double val[4], d3[4], lf[4], lv[4], ld[4];
 #pragma unroll
       for (int v = 0; v < kNv; ++v)
@@ -475,8 +483,10 @@ int launch_adj_dir(const AdjDirArgs& a, Grid g, int members, hipStream_t st) {
     nline = (g.nx + CW - 1) / CW;
     nreal = (long long)nline * ((g.nz + C - 1) / C);
   }
-  if (nreal > (1LL << 31) - 1) return PMW_EINVAL;
-  k_adj_dir<DIR, CW, NT><<<dim3((unsigned)nreal, members), NT, 0, st>>>(a, g, nline, (int)nreal);
+  if (nreal > (1LL << 31) - 64) return PMW_EINVAL;
+  const int per = (int)((nreal + 7) / 8);
+  k_adj_dir<DIR, CW, NT><<<dim3((unsigned)per * 8, members), NT, 0, st>>>(a, g, nline,
+                                                                          (int)nreal, per);
   PMW_CHECK_LAUNCH();
   return PMW_OK;
 }

@@ -28,7 +28,7 @@ def checker(tmp_path_factory):
 def _run(exe, n, lo, hi, y):
     out = subprocess.run([exe, str(n), str(lo), str(hi), y], capture_output=True, text=True,
                          check=True).stdout.split()
-    return [int(v) for v in out]
+    return [int(v) for v in out[:5]] + [float(out[5])]
 
 
 @pytest.mark.parametrize("lo,hi", [(50.0, 2000.0), (1e-3, 1e5), (1e-200, 1e200)])
@@ -36,15 +36,18 @@ def test_pow_gamma_correctly_rounded_almost_always(checker, lo, hi):
     """x^gamma over the range rho*theta takes (and far beyond): the result
     is the correctly rounded one in all but <= 2e-5 of the arguments (glibc
     misses ~8e-4 in the physical range); +-0, +-inf, -1, NaN behave as pow."""
-    n, bad_cr, _, glibc_cr, special = _run(checker, 200000, lo, hi, GAMMA)
+    n, bad_cr, _, glibc_cr, special, _ = _run(checker, 200000, lo, hi, GAMMA)
     assert special == 0
     assert bad_cr <= 2e-5 * n, (bad_cr, glibc_cr)
 
 
 def test_pow_gamma_minus_one(checker):
-    """y = gamma - 1 (dp/dq of the adjoint)."""
-    n, bad_cr, _, _, special = _run(checker, 100000, 50.0, 2000.0, repr(float(GAMMA) - 1.0))
+    """y = gamma - 1 (dp/dq of the adjoint): pow_pos as above, and the
+    adjoint's few-ulp pow_pos_fast within 5e-15 relative."""
+    n, bad_cr, _, _, special, fast_err = _run(checker, 100000, 1e-3, 1e5,
+                                              repr(float(GAMMA) - 1.0))
     assert special == 0 and bad_cr <= 2e-5 * n
+    assert fast_err <= 5e-15, fast_err
 
 
 def test_tables_regenerate_identically():
