@@ -236,8 +236,9 @@ int pmw_ensemble_prepare(const pmw_member* members, int nmembers, const int* tun
                          void* workspace, long long workspace_bytes, void* stream);
 
 /* evolve (pyminiweather/solve/step.py:85-143) for every member at once:
- * `nsteps` timesteps, each one persistent launch per direction covering all
- * members' tiles (the three RK stages fused as in pmw_evolve_ex). `members`
+ * `nsteps` timesteps, each one launch per direction over the concatenated
+ * tile lists of all members (the three RK stages fused as in
+ * pmw_evolve_ex). `members`
  * and `tuning` must be what was prepared. Only kernel launches are issued,
  * so a call can be captured in a HIP graph. Each member ends bit-identical
  * to pmw_evolve on that member alone. kernel_ms as in pmw_evolve_ex (each

@@ -19,13 +19,15 @@
 //  * x (k_dir_x<NT>): one block of NT threads per (row, tile of NT-12 cells).
 //    S, the running stage state and face fluxes live in LDS; periodic wrap
 //    replaces the halo columns at every stage.
-//  * z (k_dir_z<L>): one lane per column marching a segment of L rows through
-//    a three-level register pipeline: at step j stage 1 takes S row j and
-//    finishes face j-1 and cell j-2, stage 2 finishes face j-3 and cell j-4,
-//    stage 3 face j-5 and cell j-6. Wall halo rows of each stage's forcing
-//    are synthesised in registers from that stage's edge row as it appears.
-//    No barriers. Extra lanes past nx write the corner columns of the halo
-//    rows. The bottom / top segment writes the z halo rows of st and tmp.
+//  * z, LDS tiles (k_dir_zt<CW,NT>, the default): the x scheme turned on its
+//    side, CW columns x NT/CW rows per block, the wall halo rows of every
+//    stage state rebuilt from its edge row.
+//  * z, marching (k_dir_z<L>): a block is three waves over 64 columns and a
+//    segment of L rows; wave s runs RK stage s+1 one row step behind the
+//    previous stage, rows handed over through a 2-slot LDS ring with one
+//    barrier per step. Wall halo rows are synthesised in registers. Extra
+//    lanes past nx write the corner columns of the halo rows.
+//  * k_ens_*: the same tiles over the flattened tile list of an ensemble.
 // Block ids are remapped so that each XCD runs a contiguous range of tiles.
 #include "pmw_common.h"
 #include "pmw_fused.h"

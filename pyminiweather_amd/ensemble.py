@@ -1,8 +1,9 @@
 """Ensembles of uneven grids advanced together (batched form).
 
 Each member is an ordinary ``Fields`` (its own grid size, timestep, IC);
-``Ensemble.evolve`` advances all of them with one persistent launch per
-direction and timestep (``pmw_ensemble_evolve``), which leaves every member
+``Ensemble.evolve`` advances all of them with one launch per direction and
+timestep over the concatenated tile lists of all members
+(``pmw_ensemble_evolve``), which leaves every member
 bit-identical to ``solve.evolve`` on that member alone. ``capture`` records
 a whole number of timestep pairs in a HIP graph (torch.cuda.CUDAGraph) so
 that small members are not bound by launch latency.

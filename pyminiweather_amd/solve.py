@@ -11,9 +11,11 @@ ctypes call into libpmw.so:
   ``state_out`` overlaps ``state_forcing`` (the reference's aliased stage 2)
   it runs the per-function kernels instead, whose tendency is complete
   before the update, exactly like the reference.
-* ``evolve`` (solve/step.py:85-143): six fused launches. Stage 1 writes
-  ``fields.state_work`` so no launch reads and writes one buffer; ``state``
-  and ``state_tmp`` end up exactly as the reference leaves them.
+* ``evolve`` (solve/step.py:85-143): one launch per direction, each running
+  the three RK stages of that direction in LDS tiles (per-stage launches
+  for the injection BC, or when chosen by the tuning). The first direction
+  writes ``fields.state_work`` so no launch reads and writes one buffer;
+  ``state`` and ``state_tmp`` end up exactly as the reference leaves them.
 
 The fused paths do not materialise ``vals_*``, ``d3_vals_*``, ``flux`` or
 ``tend``; those are scratch in the reference too (every substep overwrites
