@@ -19,9 +19,9 @@ from . import solve as S
 from ._lib import ptr, stream_of
 
 X_BLOCKS = (256, 448, 512)
-# z: rows per marching segment, or an LDS tile code (8064: 8 columns x 64
-# rows, 4128: 4 columns x 128 rows)
-Z_SEGMENTS = (8, 16, 24, 32, 48, 64, 8064, 4128)
+# z: rows per marching segment, or an LDS tile code columns*1000 + rows
+# (8064, 4128: one cell per thread; 8128: two cells per thread)
+Z_SEGMENTS = (8, 16, 24, 32, 48, 64, 8064, 4128, 8128)
 
 
 def _time_grid(params, fields, tuning, steps, bufs):

@@ -610,6 +610,160 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
                       logical == 0);
 }
 
+// z tiles of CW columns x NR rows with two cells per thread (tile rows j
+// and j + NR/2 of one column) and the stage state updated in place: LDS holds
+// the running stage state and the face fluxes only (the RK init S of a
+// thread's two cells stays in registers), 64 B per cell, so an 8 x 128 tile
+// fits in 64 KB. Same operations in the same order as dir_zt_tile.
+template <int NC>
+struct ZTile2Lds {
+  double s[kNv][NC];
+  double fl[kNv][NC];
+};
+
+template <int CW, int NR>
+__device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, int c0, int r0,
+                                             ZTile2Lds<CW * NR>& sm, bool corners) {
+  constexpr int NC = CW * NR, NT = NC / 2, H = 6, C = NR - 2 * H;
+  auto& s = sm.s;
+  auto& fl = sm.fl;
+  const int nx = g.nx, nz = g.nz, t = threadIdx.x;
+  const int col = t % CW;
+  const int i = c0 + col;
+  const bool colok = i < nx;
+  const int pc = i + kHs;
+  const gdouble* hy = a.hy_cell;
+  const int plo = kHs, phi = nz + kHs - 1;
+  const bool own_lo = r0 == 0, own_hi = r0 <= nz - 1 && nz - 1 < r0 + C;
+  int jj[2], kk[2], sl[2], es[2];
+  bool wl[2], wh[2], own[2];
+  double S[2][4], src[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    sl[c] = t + c * NT;
+    jj[c] = sl[c] / CW;
+    kk[c] = r0 - H + jj[c];
+    wl[c] = kk[c] == -2 || kk[c] == -1;
+    wh[c] = kk[c] == nz || kk[c] == nz + 1;
+    own[c] = colok && kk[c] >= r0 && kk[c] < r0 + C && kk[c] < nz;
+    es[c] = wl[c] ? sl[c] - kk[c] * CW : sl[c] - (kk[c] - (nz - 1)) * CW;
+    double q[4] = {0.0, 0.0, 0.0, 0.0};
+    if (colok) {
+      if (kk[c] >= 0 && kk[c] < nz) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) q[v] = a.in[g.at(v, kk[c] + kHs, pc)];
+      } else if (wl[c] || wh[c]) {
+        const int pe = wl[c] ? plo : phi;
+#pragma unroll
+        for (int v = 0; v < kNv; ++v)
+          q[v] = wall_halo(v, a.in[g.at(v, pe, pc)], hy[pe], hy[kk[c] + kHs]);
+        if ((wl[c] && own_lo) || (wh[c] && own_hi)) {  // set_bc_z of stage 1 into st
+#pragma unroll
+          for (int v = 0; v < kNv; ++v) a.st[g.at(v, kk[c] + kHs, pc)] = q[v];
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) {
+      s[v][sl[c]] = q[v];
+      S[c][v] = q[v];
+    }
+    src[c] = (a.src_w && colok && kk[c] >= 0 && kk[c] < nz)
+                 ? a.src_w[(long long)kk[c] * nx + i] : 0.0;
+  }
+  if (corners && t < 8) {
+    // corner columns (padded 0, 1, nx+2, nx+3) of the halo rows of st, tmp
+    gdouble* buf = t < 4 ? a.st : a.tmp;
+    const int qq = t & 3;
+    const int cc = qq < 2 ? qq : nx + qq;
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) {
+      const double e = buf[g.at(v, plo, cc)];
+      buf[g.at(v, 0, cc)] = wall_halo(v, e, hy[plo], hy[0]);
+      buf[g.at(v, 1, cc)] = wall_halo(v, e, hy[plo], hy[1]);
+      const double f = buf[g.at(v, phi, cc)];
+      buf[g.at(v, nz + 2, cc)] = wall_halo(v, f, hy[phi], hy[nz + 2]);
+      buf[g.at(v, nz + 3, cc)] = wall_halo(v, f, hy[phi], hy[nz + 3]);
+    }
+  }
+  const double id = 1.0 / a.d;
+  __syncthreads();
+#pragma unroll
+  for (int stage = 0; stage < 3; ++stage) {
+    const int e = 2 + 2 * stage;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int j = jj[c], k = kk[c], u = sl[c];
+      if (colok && j >= e && j <= NR - e && k >= 0 && k <= nz) {
+        double v4[4], d3[4], G[4];
+#pragma unroll
+        for (int v = 0; v < kNv; ++v)
+          recon4(s[v][u - 2 * CW], s[v][u - CW], s[v][u], s[v][u + CW], v4[v], d3[v]);
+        flux_z_face(v4, d3, a.hyr_int[k], a.hyt_int[k], a.hyp_int[k], a.hv, k == 0 || k == nz,
+                    G);
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) fl[v][u] = G[v];
+      }
+    }
+    __syncthreads();
+    const double dts = stage == 0 ? a.dt1 : stage == 1 ? a.dt2 : a.dt3;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int j = jj[c], k = kk[c], u = sl[c];
+      if (colok && j >= e && j <= NR - e - 1 && k >= 0 && k < nz) {
+        double td[4];
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) td[v] = tend_of(fl[v][u], fl[v][u + CW], a.d, id);
+        td[2] = td[2] - s[0][u] * kGrav;
+        if (a.src_w) td[2] = td[2] + src[c];
+        double q[4];
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) q[v] = S[c][v] + dts * td[v];
+        if (stage < 2) {
+#pragma unroll
+          for (int v = 0; v < kNv; ++v) s[v][u] = q[v];  // in place: faces are done
+        }
+        if (own[c]) {
+          gdouble* dst = stage == 0 ? a.s1 : stage == 1 ? a.tmp : a.out;
+          if (dst) {
+#pragma unroll
+            for (int v = 0; v < kNv; ++v) dst[g.at(v, k + kHs, pc)] = q[v];
+          }
+        }
+      }
+    }
+    if (stage == 2) break;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (colok && (wl[c] || wh[c])) {
+        const int pe = wl[c] ? plo : phi;
+        double q[4];
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) q[v] = wall_halo(v, s[v][es[c]], hy[pe], hy[kk[c] + kHs]);
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) s[v][sl[c]] = q[v];
+        if (stage == 1 && ((wl[c] && own_lo) || (wh[c] && own_hi))) {  // set_bc_z of stage 3
+#pragma unroll
+          for (int v = 0; v < kNv; ++v) a.tmp[g.at(v, kk[c] + kHs, pc)] = q[v];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int CW, int NR>
+__global__ __launch_bounds__(CW * NR / 2) void k_dir_zt2(DirArgs a, Grid g, int ncolgrp,
+                                                         int nreal, int per_xcd) {
+  __shared__ ZTile2Lds<CW * NR> sm;
+  const int logical = xcd_logical(blockIdx.x, per_xcd);
+  if (logical >= nreal) return;  // whole block, before any barrier
+  member_offset(a);
+  dir_zt2_tile<CW, NR>(a, g, (logical % ncolgrp) * CW, (logical / ncolgrp) * (NR - 12), sm,
+                       logical == 0);
+}
+
 // ------------------------------------------------------------- ensemble --
 // Device descriptor of one member of an ensemble (uneven grids).
 struct MemberDev {
@@ -706,6 +860,31 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
                       (local / md.z_ncol) * C, sm, local == 0);
 }
 
+template <int CW, int NR>
+__global__ __launch_bounds__(CW * NR / 2) void k_ens_zt2(const MemberDev* __restrict__ m, int n,
+                                                         int pass, int total, int per_xcd) {
+  __shared__ ZTile2Lds<CW * NR> sm;
+  const int item = xcd_logical(blockIdx.x, per_xcd);
+  if (item >= total) return;  // whole block, before any barrier
+  const MemberDev& md = m[member_of<1>(m, n, item)];
+  const DirArgs a = member_args(md, 1, pass);
+  const int local = item - md.z_start;
+  dir_zt2_tile<CW, NR>(a, Grid(md.nx, md.nz), (local % md.z_ncol) * CW,
+                       (local / md.z_ncol) * (NR - 12), sm, local == 0);
+}
+
+template <int CW, int NR>
+int launch_zt2(const DirArgs& a, Grid g, hipStream_t st, int members) {
+  constexpr int C = NR - 12;
+  const int ncolgrp = (g.nx + CW - 1) / CW;
+  const long long nreal = (long long)ncolgrp * ((g.nz + C - 1) / C);
+  const int per = (int)((nreal + kNumXcd - 1) / kNumXcd);
+  k_dir_zt2<CW, NR><<<dim3(per * kNumXcd, members), CW * NR / 2, 0, st>>>(a, g, ncolgrp,
+                                                                          (int)nreal, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
 template <int NT>
 int launch_x(const DirArgs& a, Grid g, hipStream_t st, int members) {
   constexpr int C = NT - 12;
@@ -745,11 +924,11 @@ int launch_z(const DirArgs& a, Grid g, hipStream_t st, int members) {
 namespace pmw {
 
 bool fused_x_tile_ok(int nt) { return nt == 256 || nt == 448 || nt == 512; }
-// z: 8..64 = rows per marching segment; 8064 / 4128 = LDS tiles of 8 columns
-// x 64 rows / 4 columns x 128 rows
+// z: 8..64 = rows per marching segment; columns*1000 + rows = LDS tiles:
+// 8064, 4128 (one cell per thread), 8128 (two cells per thread)
 bool fused_z_seg_ok(int seg) {
   return seg == 8 || seg == 16 || seg == 24 || seg == 32 || seg == 48 || seg == 64 ||
-         seg == 8064 || seg == 4128;
+         seg == 8064 || seg == 4128 || seg == 8128;
 }
 
 int launch_dir_x(const DirArgs& a, Grid g, int nt, hipStream_t st, int members) {
@@ -773,6 +952,7 @@ int launch_dir_z(const DirArgs& a, Grid g, int seg, hipStream_t st, int members)
     case 64: return launch_z<64>(a, g, st, members);
     case 8064: return launch_zt<8, 512>(a, g, st, members);
     case 4128: return launch_zt<4, 512>(a, g, st, members);
+    case 8128: return launch_zt2<8, 128>(a, g, st, members);
     default: return PMW_EINVAL;
   }
 }
@@ -801,8 +981,8 @@ int plan_members(const pmw_member* mem, int n, int x_nt, int z_seg, MemberDev* d
       return PMW_EALIAS;
     const int xt = (m.nx + ens_x_cells(x_nt) - 1) / ens_x_cells(x_nt);
     int zc, zsg;
-    if (z_seg >= 1000) {  // LDS tiles: CW columns x (512 / CW) rows
-      const int cw = z_seg / 1000, rows = 512 / cw - 12;
+    if (z_seg >= 1000) {  // LDS tiles: columns*1000 + rows (12 of them halo)
+      const int cw = z_seg / 1000, rows = z_seg % 1000 - 12;
       zc = (m.nx + cw - 1) / cw;
       zsg = (m.nz + rows - 1) / rows;
     } else {
@@ -869,6 +1049,14 @@ int launch_ens_z(const MemberDev* d, int n, int pass, int total, hipStream_t st)
   return PMW_OK;
 }
 
+template <int CW, int NR>
+int launch_ens_zt2(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
+  const int per = ens_per_xcd(total);
+  k_ens_zt2<CW, NR><<<per * kNumXcd, CW * NR / 2, 0, st>>>(d, n, pass, total, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
 template <int CW, int NT>
 int launch_ens_zt(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
   const int per = ens_per_xcd(total);
@@ -892,6 +1080,7 @@ int launch_ens(int dir, int x_nt, int z_seg, const MemberDev* d, int n, int pass
     case 48: return launch_ens_z<48>(d, n, pass, plan.z_total, st);
     case 8064: return launch_ens_zt<8, 512>(d, n, pass, plan.z_total, st);
     case 4128: return launch_ens_zt<4, 512>(d, n, pass, plan.z_total, st);
+    case 8128: return launch_ens_zt2<8, 128>(d, n, pass, plan.z_total, st);
     default: return launch_ens_z<64>(d, n, pass, plan.z_total, st);
   }
 }
