@@ -601,9 +601,9 @@ namespace {
 struct Tuning {
   int mode;   // 0: one fused launch per direction; 1: one launch per RK stage
   int x_nt;   // fused x block size (256, 448 or 512 threads; tile = x_nt - 12 cells)
-  int z_seg;  // fused z: rows per marching segment (8..64) or LDS tile code (8064, 4128)
+  int z_seg;  // fused z: rows per marching segment (8..64) or LDS tile code (8064, 4128, 8128)
 };
-constexpr Tuning kDefaultTuning = {0, 512, 4128};
+constexpr Tuning kDefaultTuning = {0, 512, 8128};
 
 int tuning_from(const int* t, Tuning& out) {
   out = kDefaultTuning;
