@@ -242,7 +242,7 @@ def bench_batched_adjoint(members=64, nsteps=400, iters=20):
     forward run from the thermal IC + N(0, 0.1 K) theta' noise (numpy
     default_rng(2 + m)); `iters` steepest-descent iterations on the initial
     theta', forward and checkpointed adjoint batched in one launch per
-    direction (forward) / stage (adjoint) for all members."""
+    direction for all members."""
     import numpy as np
     import torch
     from pyminiweather_amd.adjoint import AdjointSolver
@@ -265,8 +265,8 @@ def bench_batched_adjoint(members=64, nsteps=400, iters=20):
            "unit": "fwd+adjoint member-cell-steps/s",
            "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
            "adjoint_cost_vs_forward": t_bwd / t_fwd,
-           "launch": "one launch per direction (forward) and per RK stage (adjoint) "
-                     "for all members"}
+           "launch": "one launch per direction for all members, forward and adjoint "
+                     "(the adjoint kernel runs a direction's three stage adjoints)"}
     if iters:
         (_, hist), t_all = _timed(lambda: solver.assimilate(theta0, target, iters=iters))
         red = hist[-1] / hist[0]
