@@ -8,8 +8,11 @@ The backward sweep takes the segments in reverse: each is recomputed from
 its checkpoint into a tape that holds every stage state of the segment
 (``pmw_tape_forward``), then swept backwards with the hand-written adjoint
 kernels (``pmw_tape_adjoint``: one launch per direction, all three RK
-stages, for all members).
-Memory is O(sqrt(N)) states plus one segment's tape (6 states per step);
+stages, for all members). Two tapes alternate between segments, so the
+recompute of segment s-1 runs on a second stream while the adjoint of
+segment s runs on the caller's stream (events order each adjoint after its
+recompute, and each recompute after the adjoint that last read its tape).
+Memory is O(sqrt(N)) states plus two segments' tapes (6 states per step);
 the recompute is one extra forward run.
 
 ``EvolveFunction`` wraps the same as a ``torch.autograd.Function`` (initial
@@ -60,7 +63,12 @@ class AdjointSolver:
         self.bshape = (self.members,) + tuple(fields.shape)
         dev = fields.device
         n = min(self.every, self.nsteps)
-        self._tape = torch.zeros((6 * n + 1,) + self.bshape, dtype=torch.float64, device=dev)
+        nseg = -(-self.nsteps // self.every)
+        self._tapes = [torch.zeros((6 * n + 1,) + self.bshape, dtype=torch.float64, device=dev)
+                       for _ in range(min(nseg, 2))]
+        self._tape = self._tapes[0]
+        # recompute stream (default priority), used when there are >= 2 segments
+        self._side = torch.cuda.Stream(device=dev) if nseg > 1 else None
         self._ws = torch.zeros((1,) + self.bshape, dtype=torch.float64, device=dev)
         self.adjoint_tuning = ADJ_DEFAULT_TUNING
         self._adj_tune = (ctypes.c_int * 2)(*self.adjoint_tuning)
@@ -71,20 +79,22 @@ class AdjointSolver:
     def _batch(self, x):
         return x.reshape(self.bshape)
 
-    def _tape_run(self, start, first_step, n, store):
+    def _tape_run(self, start, first_step, n, store, tape=None):
         """Tape slot 0 <- start; n steps from step index first_step (the
         direction order of step k is Z-first for even k, as the reference's
-        toggle gives in a fresh run). Returns the end-state slot (a view)."""
+        toggle gives in a fresh run), on the current stream. Returns the
+        end-state slot (a view)."""
         f = self.f
-        self._tape[0].copy_(start)
+        tape = self._tape if tape is None else tape
+        tape[0].copy_(start)
         tune = (ctypes.c_int * 3)(*self.tuning)
-        _lib.call("pmw_tape_forward", ptr(self._tape), self._tape.numel(), self.members,
+        _lib.call("pmw_tape_forward", ptr(tape), tape.numel(), self.members,
                   ptr(f.hy_dens_cell), ptr(f.hy_dens_theta_cell), ptr(f.hy_dens_int),
                   ptr(f.hy_dens_theta_int), ptr(f.hy_pressure_int),
                   ptr(f.src_w) if f.src_w is not None else None, f.nx, f.nz,
                   float(self.p["dx"]), float(self.p["dz"]), float(self.p["dt"]), self._hvx,
                   self._hvz, first_step % 2, int(n), int(store), tune, stream_of(f.device))
-        return self._tape[6 * n]
+        return tape[6 * n]
 
     # -- forward / backward ------------------------------------------------
     def forward(self, state0):
@@ -100,21 +110,54 @@ class AdjointSolver:
             done += n
         return s.clone().reshape(state0.shape), ckpts
 
+    def _recompute(self, ckpts, seg):
+        start = seg * self.every
+        n = min(self.every, self.nsteps - start)
+        self._tape_run(ckpts[seg], start, n, store=True, tape=self._tapes[seg % 2])
+
+    def _adjoint(self, lam, seg):
+        f = self.f
+        start = seg * self.every
+        n = min(self.every, self.nsteps - start)
+        tape = self._tapes[seg % 2]
+        _lib.call("pmw_tape_adjoint", ptr(tape), tape.numel(), self.members,
+                  ptr(lam), ptr(self._ws), self._ws.numel(), ptr(f.hy_dens_cell),
+                  ptr(f.hy_dens_theta_cell), ptr(f.hy_dens_int),
+                  ptr(f.hy_dens_theta_int), f.nx, f.nz, float(self.p["dx"]),
+                  float(self.p["dz"]), float(self.p["dt"]), self._hvx, self._hvz,
+                  start % 2, int(n), self._adj_tune, stream_of(f.device))
+
     def backward(self, lam_final, ckpts):
         """dL/d(initial states) (interiors; state layout) given dL/d(final
-        states) (interior of lam_final)."""
-        f = self.f
+        states) (interior of lam_final). Ordered on the caller's stream: on
+        return, that stream follows every launch made here."""
         lam = self._batch(lam_final).clone()
-        for seg in reversed(range(len(ckpts))):
-            start = seg * self.every
-            n = min(self.every, self.nsteps - start)
-            self._tape_run(ckpts[seg], start, n, store=True)
-            _lib.call("pmw_tape_adjoint", ptr(self._tape), self._tape.numel(), self.members,
-                      ptr(lam), ptr(self._ws), self._ws.numel(), ptr(f.hy_dens_cell),
-                      ptr(f.hy_dens_theta_cell), ptr(f.hy_dens_int),
-                      ptr(f.hy_dens_theta_int), f.nx, f.nz, float(self.p["dx"]),
-                      float(self.p["dz"]), float(self.p["dt"]), self._hvx, self._hvz,
-                      start % 2, int(n), self._adj_tune, stream_of(f.device))
+        nseg = len(ckpts)
+        if nseg == 1:
+            self._recompute(ckpts, 0)
+            self._adjoint(lam, 0)
+            return lam.reshape(lam_final.shape)
+        main = torch.cuda.current_stream(self.f.device)
+        side = self._side
+        side.wait_stream(main)  # checkpoints and lam are ready
+        ready = {}
+
+        def recompute_on_side(seg):
+            with torch.cuda.stream(side):
+                self._recompute(ckpts, seg)
+                ready[seg] = torch.cuda.Event()
+                ready[seg].record(side)
+
+        recompute_on_side(nseg - 1)
+        recompute_on_side(nseg - 2)
+        for seg in reversed(range(nseg)):
+            main.wait_event(ready.pop(seg))
+            self._adjoint(lam, seg)
+            if seg >= 2:  # tape seg % 2 is free once this adjoint is done
+                done = torch.cuda.Event()
+                done.record(main)
+                side.wait_event(done)
+                recompute_on_side(seg - 2)
         return lam.reshape(lam_final.shape)
 
     # -- the theta' assimilation problem ------------------------------------

@@ -191,6 +191,27 @@ def test_checkpoint_interval_does_not_change_gradient():
     np.testing.assert_array_equal(grads[0], grads[2])
 
 
+def test_overlapped_backward_bitwise_equals_serial():
+    """Six segments (recompute of segment s-1 on the second stream while the
+    adjoint of segment s runs) against one segment (no second stream), on a
+    grid large enough that the two streams' kernels really run together."""
+    from pyminiweather_amd.adjoint import AdjointSolver
+    p = O.make_params(400, 200, "thermal", cfl=1.5, max_speed=450.0)
+    f = _fields(p)
+    g = torch.Generator(device="cuda").manual_seed(17)
+    theta = torch.randn((2, 200, 400), dtype=torch.float64, device="cuda", generator=g)
+    target = torch.randn((2, 200, 400), dtype=torch.float64, device="cuda", generator=g)
+    serial = AdjointSolver(p, f, 12, checkpoint_every=12, members=2)
+    assert serial._side is None
+    lo, go = serial.loss_and_grad(theta, target)
+    overlapped = AdjointSolver(p, f, 12, checkpoint_every=2, members=2)
+    assert overlapped._side is not None and len(overlapped._tapes) == 2
+    for _ in range(2):
+        lt, gt = overlapped.loss_and_grad(theta, target)
+        np.testing.assert_array_equal(lo.cpu().numpy(), lt.cpu().numpy())
+        np.testing.assert_array_equal(go.cpu().numpy(), gt.cpu().numpy())
+
+
 def test_autograd_function_composes():
     """EvolveFunction inside a torch graph gives torch.autograd the HIP
     adjoint: the gradient of sum(final^2) w.r.t. the initial interior equals
