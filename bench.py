@@ -200,8 +200,10 @@ def bench_adjoint(nsteps=400, fd_cells=8):
     import numpy as np
     import torch
     from pyminiweather_amd.adjoint import AdjointSolver
+    from pyminiweather_amd.autotune import tune_tape
     p, f = _thermal_400x200()
     solver = AdjointSolver(p, f, nsteps)
+    tune_tape(solver)
     w = solver.row_weight()[:, None]
     theta0 = (f.state[3, 2:-2, 2:-2] / w).clone()
     noise = torch.from_numpy(np.random.default_rng(1).normal(0.0, 0.1, (200, 400))).cuda()
@@ -246,8 +248,10 @@ def bench_batched_adjoint(members=64, nsteps=400, iters=20):
     import numpy as np
     import torch
     from pyminiweather_amd.adjoint import AdjointSolver
+    from pyminiweather_amd.autotune import tune_tape
     p, f = _thermal_400x200()
     solver = AdjointSolver(p, f, nsteps, members=members)
+    tune_tape(solver)
     w = solver.row_weight()[:, None]
     theta0 = (f.state[3, 2:-2, 2:-2] / w).expand(members, 200, 400).clone()
     noise = np.stack([np.random.default_rng(2 + m).normal(0.0, 0.1, (200, 400))
@@ -266,7 +270,8 @@ def bench_batched_adjoint(members=64, nsteps=400, iters=20):
            "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
            "adjoint_cost_vs_forward": t_bwd / t_fwd,
            "launch": "one launch per direction for all members, forward and adjoint "
-                     "(the adjoint kernel runs a direction's three stage adjoints)"}
+                     "(the adjoint kernel runs a direction's three stage adjoints)",
+           "tuning": list(solver.tuning), "adjoint_tuning": list(solver.adjoint_tuning)}
     if iters:
         (_, hist), t_all = _timed(lambda: solver.assimilate(theta0, target, iters=iters))
         red = hist[-1] / hist[0]

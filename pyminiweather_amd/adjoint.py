@@ -52,7 +52,13 @@ class AdjointSolver:
         if self.nsteps < 1:
             raise ValueError("nsteps must be >= 1")
         self.every = int(checkpoint_every or max(1, math.ceil(math.sqrt(self.nsteps))))
-        tun = tuple(tuning) if tuning is not None else S.tuning_for(fields.nx, fields.nz)
+        if tuning is not None:
+            tun = tuple(tuning)
+        elif self.members > 1:
+            # one launch covers every member: the shapes tuned for large launches
+            tun = S.DEFAULT_TUNING
+        else:
+            tun = S.tuning_for(fields.nx, fields.nz)
         if tun[0] != 0:
             tun = S.DEFAULT_TUNING
         self.tuning = tun

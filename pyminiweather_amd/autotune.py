@@ -83,3 +83,38 @@ def tune_ensemble(ens, steps=2):
     best = (0, best_x, best_z)
     ens.set_tuning(best)
     return best, table
+
+
+def tune_tape(solver, steps=2):
+    """Pick the fused (x block, z kernel) for an AdjointSolver's tape forward,
+    where one launch covers every member (the best shape for a batch of
+    small grids is not the one for the same grid alone). Each candidate is
+    timed with HIP events around whole tape-forward calls, run on the
+    solver's first tape as scratch, from its fields' state in every member;
+    z first (x = 512), then x with the best z. Sets solver.tuning and
+    returns (tuning, table of ms per call)."""
+    f = solver.f
+    steps = max(1, min(int(steps), solver.every))
+    start = f.state.unsqueeze(0).expand(solver.bshape)
+    table = {}
+
+    def timed(t):
+        solver.tuning = t
+        solver._tape_run(start, 0, steps, store=False)  # warm-up
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        solver._tape_run(start, 0, steps, store=False)
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1)
+
+    for zs in Z_SEGMENTS:
+        table[(0, 512, zs)] = timed((0, 512, zs))
+    best_z = min(Z_SEGMENTS, key=lambda zs: table[(0, 512, zs)])
+    for xb in X_BLOCKS:
+        if (0, xb, best_z) not in table:
+            table[(0, xb, best_z)] = timed((0, xb, best_z))
+    best = min(((0, xb, best_z) for xb in X_BLOCKS), key=lambda t: table[t])
+    solver.tuning = best
+    return best, table

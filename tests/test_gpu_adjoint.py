@@ -212,6 +212,24 @@ def test_overlapped_backward_bitwise_equals_serial():
         np.testing.assert_array_equal(go.cpu().numpy(), gt.cpu().numpy())
 
 
+def test_tape_autotune_keeps_gradient_bitwise():
+    """tune_tape picks a valid fused shape for the batch, and every shape
+    gives the same gradient bit for bit."""
+    from pyminiweather_amd.adjoint import AdjointSolver
+    from pyminiweather_amd.autotune import X_BLOCKS, Z_SEGMENTS, tune_tape
+    p = O.make_params(100, 50, "thermal", cfl=1.5, max_speed=450.0)
+    f = _fields(p)
+    g = torch.Generator(device="cuda").manual_seed(23)
+    theta = torch.randn((3, 50, 100), dtype=torch.float64, device="cuda", generator=g)
+    target = torch.randn((3, 50, 100), dtype=torch.float64, device="cuda", generator=g)
+    solver = AdjointSolver(p, f, 6, members=3)
+    ref = solver.loss_and_grad(theta, target)[1].cpu().numpy()
+    best, table = tune_tape(solver)
+    assert best == solver.tuning and best[0] == 0
+    assert best[1] in X_BLOCKS and best[2] in Z_SEGMENTS and best in table
+    np.testing.assert_array_equal(ref, solver.loss_and_grad(theta, target)[1].cpu().numpy())
+
+
 def test_autograd_function_composes():
     """EvolveFunction inside a torch graph gives torch.autograd the HIP
     adjoint: the gradient of sum(final^2) w.r.t. the initial interior equals
