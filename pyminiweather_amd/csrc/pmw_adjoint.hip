@@ -317,12 +317,16 @@ struct AdjDirArgs {
   long long mstride;
 };
 
+// la holds lam3, then lamS2, then lamS1, each stage's result written in
+// place after its faces are done: a stage's faces read only slots the stage
+// before wrote (faces 2+2m..NR-2-2m read slots 1+2m..NR-2-2m, inside the
+// cells 3+2(m-1)..NR-4-2(m-1) of stage m-1), and slots outside the domain
+// keep the zero they were loaded with.
 template <int NT>
 struct AdjDirLds {
   double st[kNv][NT];  // forcing state of the stage being reversed
   double fw[8][NT];    // face cotangents of (vals, d3)
-  double la[kNv][NT];  // lam3, then lamS1
-  double lb[kNv][NT];  // lamS2
+  double la[kNv][NT];  // lam3, then lamS2, then lamS1
 };
 
 // Consecutive tiles (neighbouring columns / row segments) go to the same XCD
@@ -368,7 +372,6 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
   for (int v = 0; v < kNv; ++v) {
     l3[v] = inside ? a.lam[off + g.at(v, prow, pcol)] : 0.0;
     sm.la[v][t] = l3[v];
-    sm.lb[v][t] = 0.0;
   }
   auto load_state = [&](const double* src) {
 #pragma unroll
@@ -386,7 +389,7 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
 #pragma unroll
   for (int m = 0; m < 3; ++m) {  // m = 0: stage 3 (forcing S2), 1: stage 2, 2: stage 1
     const int stage = 2 - m;
-    double (*lin)[NT] = m == 1 ? sm.lb : sm.la;
+    double (*lin)[NT] = sm.la;
     const double sc = a.dts[stage];
     const double sd = sc * a.id;
     // faces at slots 2+2m .. NR-2-2m (z: faces outside 0..nz are zero)
@@ -450,10 +453,9 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
       // next forcing state; st is free once the faces are done
       load_state(a.s[stage - 1]);
       if (cell) {
-        double (*lout)[NT] = m == 0 ? sm.lb : sm.la;
 #pragma unroll
         for (int v = 0; v < kNv; ++v) {
-          lout[v][t] = l[v];
+          sm.la[v][t] = l[v];  // in place: this stage's faces are done
           if (m == 0) l2[v] = l[v]; else l1[v] = l[v];
         }
       }
