@@ -37,12 +37,15 @@ from . import solve as S
 from ._lib import ptr, stream_of
 
 RHOT = 3
-# adjoint kernels: x block threads, z tile (columns * 1000 + threads)
+# adjoint kernels: x block threads (256, 384), z tile (columns * 1000 +
+# threads: 4256, 4384, 2256, 8512)
 ADJ_DEFAULT_TUNING = (256, 4256)
+ADJ_TUNINGS = ((256, 4256), (384, 4384), (256, 2256), (256, 8512))
 
 
 class AdjointSolver:
-    def __init__(self, params, fields, nsteps, checkpoint_every=None, tuning=None, members=1):
+    def __init__(self, params, fields, nsteps, checkpoint_every=None, tuning=None, members=1,
+                 adjoint_tuning=None):
         if params["ic_type"] == "injection":
             raise ValueError("the adjoint supports the periodic-x ICs only")
         self.p = dict(params)
@@ -76,7 +79,7 @@ class AdjointSolver:
         # recompute stream (default priority), used when there are >= 2 segments
         self._side = torch.cuda.Stream(device=dev) if nseg > 1 else None
         self._ws = torch.zeros((1,) + self.bshape, dtype=torch.float64, device=dev)
-        self.adjoint_tuning = ADJ_DEFAULT_TUNING
+        self.adjoint_tuning = tuple(adjoint_tuning or ADJ_DEFAULT_TUNING)
         self._adj_tune = (ctypes.c_int * 2)(*self.adjoint_tuning)
         self._hvx = S.hv_coeff(self.p["dx"], self.p["dt"])
         self._hvz = S.hv_coeff(self.p["dz"], self.p["dt"])

@@ -385,6 +385,16 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
     }
   };
   load_state(a.s[2]);
+  // hydrostatic values of this thread's face (z) or row (x): the same for
+  // the three stages, loaded once
+  double hy_r = 0.0, hy_t = 0.0;
+  if (DIR == 0) {
+    hy_r = a.hy_cell[prow];
+    hy_t = a.hyt_cell[prow];
+  } else if (pcol - kHs < nx && p >= 0 && p <= nz) {
+    hy_r = a.hyr_int[p];
+    hy_t = a.hyt_int[p];
+  }
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < 3; ++m) {  // m = 0: stage 3 (forcing S2), 1: stage 2, 2: stage 1
@@ -405,9 +415,9 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
 #pragma unroll
       for (int v = 0; v < kNv; ++v) lf[v] = sd * (lin[v][t] - lin[v][t - S]);
       if (DIR == 0)
-        adj_flux_x(val, a.hy_cell[prow], a.hyt_cell[prow], a.hv, lf, lv, ld);
+        adj_flux_x(val, hy_r, hy_t, a.hv, lf, lv, ld);
       else
-        adj_flux_z(val, a.hyr_int[p], a.hyt_int[p], a.hv, p == 0 || p == nz, lf, lv, ld);
+        adj_flux_z(val, hy_r, hy_t, a.hv, p == 0 || p == nz, lf, lv, ld);
 #pragma unroll
       for (int v = 0; v < kNv; ++v) {
         sm.fw[v][t] = lv[v];
@@ -591,7 +601,8 @@ int pmw_tape_adjoint(const double* tape, long long tape_len, int members, double
     return PMW_EINVAL;
   const int x_nt = tuning ? tuning[0] : 256;
   const int z_tile = tuning ? tuning[1] : 4256;
-  if ((x_nt != 256 && x_nt != 384) || (z_tile != 4256 && z_tile != 4384 && z_tile != 2256))
+  if ((x_nt != 256 && x_nt != 384) ||
+      (z_tile != 4256 && z_tile != 4384 && z_tile != 2256 && z_tile != 8512))
     return PMW_EINVAL;
   if (tape_len < (6LL * nsteps + 1) * slot || workspace_len < slot) return PMW_ESPACE;
   const double* tape_end = tape + (6LL * nsteps + 1) * slot;
@@ -632,6 +643,7 @@ int pmw_tape_adjoint(const double* tape, long long tape_len, int members, double
       else
         rc = z_tile == 4256   ? launch_adj_dir<1, 4, 256>(a, g, members, st)
              : z_tile == 4384 ? launch_adj_dir<1, 4, 384>(a, g, members, st)
+             : z_tile == 8512 ? launch_adj_dir<1, 8, 512>(a, g, members, st)
                               : launch_adj_dir<1, 2, 256>(a, g, members, st);
       if (rc) return rc;
     }

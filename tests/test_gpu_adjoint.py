@@ -212,6 +212,23 @@ def test_overlapped_backward_bitwise_equals_serial():
         np.testing.assert_array_equal(go.cpu().numpy(), gt.cpu().numpy())
 
 
+def test_adjoint_tunings_bitwise_equal():
+    """Every adjoint-kernel shape (x block, z tile) gives the same gradient bit
+    for bit, on a batch whose sizes no tile divides."""
+    from pyminiweather_amd.adjoint import ADJ_TUNINGS, AdjointSolver
+    p = O.make_params(70, 45, "collision", cfl=1.5, max_speed=450.0)
+    f = _fields(p)
+    g = torch.Generator(device="cuda").manual_seed(29)
+    theta = torch.randn((2, 45, 70), dtype=torch.float64, device="cuda", generator=g)
+    target = torch.randn((2, 45, 70), dtype=torch.float64, device="cuda", generator=g)
+    grads = []
+    for t in ADJ_TUNINGS:
+        solver = AdjointSolver(p, f, 5, members=2, adjoint_tuning=t)
+        grads.append(solver.loss_and_grad(theta, target)[1].cpu().numpy())
+    for gr in grads[1:]:
+        np.testing.assert_array_equal(grads[0], gr)
+
+
 def test_tape_autotune_keeps_gradient_bitwise():
     """tune_tape picks a valid fused shape for the batch, and every shape
     gives the same gradient bit for bit."""
