@@ -292,9 +292,9 @@ int pmw_tape_forward(double* tape, long long tape_len, int members,
  * dL/d(start states) (interiors). Each step reverses its second direction,
  * then its first; each direction is one launch for all members that runs
  * the three stage adjoints (the arithmetic of pmw_adjoint_stage) in LDS.
- * workspace: 1 slot. tuning: NULL or {x block (256 | 384 threads), z tile
- * (columns*1000 + threads: 4256, 4384, 2256, 8512)}. Every choice gives the
- * same result bit for bit. No reference counterpart. */
+ * workspace: 1 slot. tuning: NULL (= {256, 8512}) or {x block (256 | 384
+ * threads), z tile (columns*1000 + threads: 4256, 4384, 2256, 8512)}. Every
+ * choice gives the same result bit for bit. No reference counterpart. */
 int pmw_tape_adjoint(const double* tape, long long tape_len, int members, double* lam,
                      double* workspace, long long workspace_len, const double* hy_dens_cell,
                      const double* hy_dens_theta_cell, const double* hy_dens_int,

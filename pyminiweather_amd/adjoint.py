@@ -39,7 +39,7 @@ from ._lib import ptr, stream_of
 RHOT = 3
 # adjoint kernels: x block threads (256, 384), z tile (columns * 1000 +
 # threads: 4256, 4384, 2256, 8512)
-ADJ_DEFAULT_TUNING = (256, 4256)
+ADJ_DEFAULT_TUNING = (256, 8512)
 ADJ_TUNINGS = ((256, 4256), (384, 4384), (256, 2256), (256, 8512))
 
 

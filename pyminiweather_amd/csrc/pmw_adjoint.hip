@@ -373,18 +373,23 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
     l3[v] = inside ? a.lam[off + g.at(v, prow, pcol)] : 0.0;
     sm.la[v][t] = l3[v];
   }
-  auto load_state = [&](const double* src) {
+  // this thread's slot of a forcing state (wall halo rows rebuilt)
+  auto fetch_state = [&](const double* src, double q[4]) {
 #pragma unroll
     for (int v = 0; v < kNv; ++v) {
-      double q = 0.0;
+      q[v] = 0.0;
       if (inside)
-        q = src[off + g.at(v, prow, pcol)];
+        q[v] = src[off + g.at(v, prow, pcol)];
       else if (wall)
-        q = wall_halo(v, src[off + g.at(v, pe, pcol)], a.hy_cell[pe], a.hy_cell[prow]);
-      sm.st[v][t] = q;
+        q[v] = wall_halo(v, src[off + g.at(v, pe, pcol)], a.hy_cell[pe], a.hy_cell[prow]);
     }
   };
-  load_state(a.s[2]);
+  {
+    double q[4];
+    fetch_state(a.s[2], q);
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) sm.st[v][t] = q[v];
+  }
   // hydrostatic values of this thread's face (z) or row (x): the same for
   // the three stages, loaded once
   double hy_r = 0.0, hy_t = 0.0;
@@ -402,6 +407,10 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
     double (*lin)[NT] = sm.la;
     const double sc = a.dts[stage];
     const double sd = sc * a.id;
+    // the next stage's forcing state, loaded now and stored to LDS once this
+    // stage's faces are done, so the loads overlap the face arithmetic
+    double nq[4];
+    if (m < 2) fetch_state(a.s[stage - 1], nq);
     // faces at slots 2+2m .. NR-2-2m (z: faces outside 0..nz are zero)
     const bool zout = DIR == 1 && (p < 0 || p > nz);
     if (zout) {
@@ -461,7 +470,8 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
     }
     if (m < 2) {
       // next forcing state; st is free once the faces are done
-      load_state(a.s[stage - 1]);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) sm.st[v][t] = nq[v];
       if (cell) {
 #pragma unroll
         for (int v = 0; v < kNv; ++v) {
@@ -600,7 +610,7 @@ int pmw_tape_adjoint(const double* tape, long long tape_len, int members, double
       !hy_dens_int || !hy_dens_theta_int || nsteps < 0 || nsteps > 100000)
     return PMW_EINVAL;
   const int x_nt = tuning ? tuning[0] : 256;
-  const int z_tile = tuning ? tuning[1] : 4256;
+  const int z_tile = tuning ? tuning[1] : 8512;
   if ((x_nt != 256 && x_nt != 384) ||
       (z_tile != 4256 && z_tile != 4384 && z_tile != 2256 && z_tile != 8512))
     return PMW_EINVAL;
