@@ -30,6 +30,11 @@ HBM_PEAK_GBS = 8000.0
 # result and state_tmp written) is measured by the rocprof counters.
 BYTES_PER_CELL_STEP = 512.0
 BYTES_PER_CELL_LAUNCH = BYTES_PER_CELL_STEP / 2.0
+# fwd+adjoint with checkpoint recompute, per cell-step: forward 512 + segment
+# recompute 512 + reverse sweep 768 (per direction, stage-3 transpose reads S2
+# and lam, writes lam2: 96 B; stage 2 reads S1, lam2, updates lam, writes
+# lam1: 160 B; stage 1 reads S, lam1, updates lam: 128 B)
+FWD_ADJ_BYTES_PER_CELL_STEP = 512.0 + 512.0 + 768.0
 
 
 def workload_params(nx=2000, nz=1000):
@@ -235,6 +240,8 @@ def bench_adjoint(nsteps=400, fd_cells=8):
             "value": cells / (t_fwd + t_bwd), "unit": "fwd+adjoint cell-steps/s",
             "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
             "adjoint_cost_vs_forward": t_bwd / t_fwd,
+            "roofline_frac": round(cells / (t_fwd + t_bwd) * FWD_ADJ_BYTES_PER_CELL_STEP
+                                   / 1e9 / HBM_PEAK_GBS, 4),
             "fd_rel_l2": fd, "fd_cells": fd_cells,
             "tuning": list(solver.tuning)}
 
@@ -269,6 +276,8 @@ def bench_batched_adjoint(members=64, nsteps=400, iters=20):
            "unit": "fwd+adjoint member-cell-steps/s",
            "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
            "adjoint_cost_vs_forward": t_bwd / t_fwd,
+           "roofline_frac": round(cells / (t_fwd + t_bwd) * FWD_ADJ_BYTES_PER_CELL_STEP
+                                   / 1e9 / HBM_PEAK_GBS, 4),
            "launch": "one launch per direction for all members, forward and adjoint "
                      "(the adjoint kernel runs a direction's three stage adjoints)",
            "tuning": list(solver.tuning), "adjoint_tuning": list(solver.adjoint_tuning)}

@@ -250,14 +250,31 @@ int pmw_ensemble_evolve(const pmw_member* members, int nmembers, const int* tuni
 
 /* ---- adjoint (reverse mode) ------------------------------------------- */
 
+/* Transpose of pmw_set_bc_x with the periodic BC (pyminiweather/ics/bcs.py:
+ * 8-39), in place on a cotangent buffer `lam` (state layout): on interior
+ * rows the cotangents of halo columns 0, 1, nx+2, nx+3 are added to the
+ * columns they were copied from (nx, nx+1, 2, 3) and set to zero; every
+ * other entry is left as is. No reference counterpart (the reference only
+ * steps forward). */
+int pmw_adjoint_set_bc_x(double* lam, int nx, int nz, void* stream);
+
+/* Transpose of pmw_set_bc_z (pyminiweather/ics/bcs.py:67-148), in place:
+ * on every column the cotangents of rows 0, 1 (nz+2, nz+3) are added to row
+ * 2 (nz+1) with the forward copy's factor (1 for rho' and (rho*theta)',
+ * hy[halo row] / hy[edge row] for rho*u, 0 for rho*w) and set to zero. No
+ * reference counterpart. */
+int pmw_adjoint_set_bc_z(double* lam, const double* hy_dens_cell, int nx, int nz,
+                         void* stream);
+
 /* Adjoint of one RK stage of discrete_step (pyminiweather/solve/step.py:
  * 21-82) in one direction (0 = x, 1 = z), excluding the init term:
  * lam_out = scale * T'(state_forcing)^T lam_in + acc1 + acc2 + acc3, where
  * T is halo + reconstruction + flux + tendency (+ gravity in z) and scale is
  * the stage dt. Cotangent buffers have the state layout; only interiors are
  * read and written. acc* may be NULL; lam_out may alias an acc but not
- * lam_in or state_forcing. Periodic x. No reference counterpart (the
- * reference only steps forward). */
+ * lam_in or state_forcing. Periodic x. With scale = 1 and no acc terms it is
+ * the adjoint of pmw_tendencies_x / pmw_tendencies_z (halo transpose
+ * included). No reference counterpart (the reference only steps forward). */
 int pmw_adjoint_stage(int direction, const double* state_forcing, const double* lam_in,
                       double scale, const double* acc1, const double* acc2,
                       const double* acc3, double* lam_out, const double* hy_dens_cell,

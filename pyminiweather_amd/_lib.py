@@ -17,6 +17,7 @@ _D = ctypes.c_double
 _LL = ctypes.c_longlong
 _IP = ctypes.POINTER(ctypes.c_int)
 
+
 class PmwMember(ctypes.Structure):
     """struct pmw_member of include/pmw.h."""
 
@@ -56,6 +57,8 @@ SIGNATURES = {
     "pmw_ensemble_prepare": (_I, [_MP, _I, _IP, _P, _LL, _P]),
     "pmw_ensemble_evolve": (_I, [_MP, _I, _IP, _P, _IP, _I, _P,
                                  ctypes.POINTER(ctypes.c_double)]),
+    "pmw_adjoint_set_bc_x": (_I, [_P, _I, _I, _P]),
+    "pmw_adjoint_set_bc_z": (_I, [_P, _P, _I, _I, _P]),
     "pmw_adjoint_stage": (_I, [_I, _P, _P, _D, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _D, _D, _P]),
     "pmw_tape_slot_len": (_LL, [_I, _I, _I]),
     "pmw_tape_forward": (_I, [_P, _LL, _I, _P, _P, _P, _P, _P, _P, _I, _I, _D, _D, _D, _D, _D,

@@ -35,12 +35,88 @@ import torch
 from . import _lib
 from . import solve as S
 from ._lib import ptr, stream_of
+from .ics import Directions, check_state
 
 RHOT = 3
 # adjoint kernels: x block threads (256, 384), z tile (columns * 1000 +
 # threads: 4256, 4384, 2256, 8512)
 ADJ_DEFAULT_TUNING = (256, 8512)
 ADJ_TUNINGS = ((256, 4256), (384, 4384), (256, 2256), (256, 8512))
+
+
+# ---- per-function adjoints ---------------------------------------------------
+# Transposes of the reference's halo, tendency and stage functions, for
+# callers that chain their own reverse pass. Cotangent buffers have the state
+# layout (4, nz+4, nx+4), float64 on the fields' device.
+
+def _periodic_only(params):
+    if params["ic_type"] == "injection":
+        raise ValueError("adjoints support the periodic-x ICs only")
+
+
+def set_bc_x_adjoint(params, fields, lam):
+    """Transpose of set_bc_x (ics/bcs.py:8-39), in place on lam: halo-column
+    cotangents are added to the columns they were copied from, then zeroed."""
+    _periodic_only(params)
+    check_state(fields, lam, "lam")
+    _lib.call("pmw_adjoint_set_bc_x", ptr(lam), fields.nx, fields.nz, stream_of(fields.device))
+
+
+def set_bc_z_adjoint(params, fields, lam):
+    """Transpose of set_bc_z (ics/bcs.py:67-148), in place on lam."""
+    check_state(fields, lam, "lam")
+    _lib.call("pmw_adjoint_set_bc_z", ptr(lam), ptr(fields.hy_dens_cell), fields.nx,
+              fields.nz, stream_of(fields.device))
+
+
+def _stage_adjoint(params, fields, direction, state_forcing, lam, scale, out):
+    _periodic_only(params)
+    check_state(fields, state_forcing, "state_forcing")
+    check_state(fields, lam, "lam")
+    check_state(fields, out, "out")
+    d = params["dx"] if direction == 0 else params["dz"]
+    _lib.call("pmw_adjoint_stage", direction, ptr(state_forcing), ptr(lam), float(scale),
+              None, None, None, ptr(out), ptr(fields.hy_dens_cell),
+              ptr(fields.hy_dens_theta_cell), ptr(fields.hy_dens_int),
+              ptr(fields.hy_dens_theta_int), fields.nx, fields.nz, float(d),
+              S.hv_coeff(d, params["dt"]), stream_of(fields.device))
+
+
+def _as_state(fields, lam_tend):
+    lam = torch.zeros(fields.shape, dtype=torch.float64, device=fields.device)
+    lam[:, 2:-2, 2:-2] = lam_tend
+    return lam
+
+
+def compute_tendencies_x_adjoint(params, fields, state_forcing, lam_tend):
+    """d<lam_tend, T_x(state_forcing)> / d state_forcing, where T_x is
+    compute_tendencies_x (halo + reconstruction + flux + tendency): returns
+    the interior cotangent as a state-layout buffer (halos zero; the halo
+    transpose is folded in). lam_tend is (4, nz, nx)."""
+    out = torch.zeros(fields.shape, dtype=torch.float64, device=fields.device)
+    _stage_adjoint(params, fields, 0, state_forcing, _as_state(fields, lam_tend), 1.0, out)
+    return out
+
+
+def compute_tendencies_z_adjoint(params, fields, state_forcing, lam_tend):
+    """The z counterpart (gravity term included; the gravity-wave source is
+    constant and has no cotangent)."""
+    out = torch.zeros(fields.shape, dtype=torch.float64, device=fields.device)
+    _stage_adjoint(params, fields, 1, state_forcing, _as_state(fields, lam_tend), 1.0, out)
+    return out
+
+
+def discrete_step_adjoint(params, fields, state_forcing, lam_out, dt, direction):
+    """Transpose of discrete_step (solve/step.py:21-82) linearised at
+    state_forcing: out = init + dt * T(forcing) gives (lam_init, lam_forcing)
+    = (lam_out, dt * T'(forcing)^T lam_out) on the interiors (state layout,
+    halos zero)."""
+    d = 0 if direction == Directions.X else 1
+    lam_init = torch.zeros(fields.shape, dtype=torch.float64, device=fields.device)
+    lam_init[:, 2:-2, 2:-2] = lam_out[:, 2:-2, 2:-2]
+    lam_forcing = torch.zeros_like(lam_init)
+    _stage_adjoint(params, fields, d, state_forcing, lam_out, dt, lam_forcing)
+    return lam_init, lam_forcing
 
 
 class AdjointSolver:

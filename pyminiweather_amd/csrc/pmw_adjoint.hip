@@ -520,9 +520,67 @@ bool grid_ok(int nx, int nz) {
 
 long long state_len(int nx, int nz) { return 4LL * (nz + 4) * (nx + 4); }
 
+// ---- halo transposes -------------------------------------------------------
+// Periodic x halos: column 0 <- nx, 1 <- nx+1, nx+2 <- 2, nx+3 <- 3 on the
+// interior rows. One thread per (variable, interior row) does all four
+// columns, so sources shared between them (nx = 2, 3) are summed in order.
+__global__ void k_adj_halo_x(double* __restrict__ l, Grid g) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= kNv * g.nz) return;
+  const int v = idx / g.nz, k = idx % g.nz, nx = g.nx;
+  double* r = l + g.at(v, k + kHs, 0);
+  const double a0 = r[0], a1 = r[1], b0 = r[nx + 2], b1 = r[nx + 3];
+  r[0] = 0.0;
+  r[1] = 0.0;
+  r[nx + 2] = 0.0;
+  r[nx + 3] = 0.0;
+  r[nx] += a0;
+  r[nx + 1] += a1;
+  r[2] += b0;
+  r[3] += b1;
+}
+
+// Wall z halos on every column: rows 0, 1 from row 2 and rows nz+2, nz+3
+// from row nz+1, each a linear map of the edge value (wall_halo: 1, the
+// density ratio, or 0), so the transpose applies the same factor.
+__global__ void k_adj_halo_z(double* __restrict__ l, Grid g, const double* __restrict__ hy) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ld) return;
+  const int nz = g.nz, lo = kHs, hi = nz + kHs - 1;
+  for (int v = 0; v < kNv; ++v) {
+    const double h0 = l[g.at(v, 0, c)], h1 = l[g.at(v, 1, c)];
+    const double h2 = l[g.at(v, nz + 2, c)], h3 = l[g.at(v, nz + 3, c)];
+    l[g.at(v, 0, c)] = 0.0;
+    l[g.at(v, 1, c)] = 0.0;
+    l[g.at(v, nz + 2, c)] = 0.0;
+    l[g.at(v, nz + 3, c)] = 0.0;
+    l[g.at(v, lo, c)] += wall_halo(v, h0, hy[lo], hy[0]) + wall_halo(v, h1, hy[lo], hy[1]);
+    l[g.at(v, hi, c)] += wall_halo(v, h2, hy[hi], hy[nz + 2]) +
+                         wall_halo(v, h3, hy[hi], hy[nz + 3]);
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int pmw_adjoint_set_bc_x(double* lam, int nx, int nz, void* stream) {
+  if (!grid_ok(nx, nz) || !lam) return PMW_EINVAL;
+  Grid g(nx, nz);
+  k_adj_halo_x<<<(unsigned)((kNv * nz + 255) / 256), 256, 0, (hipStream_t)stream>>>(lam, g);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
+int pmw_adjoint_set_bc_z(double* lam, const double* hy_dens_cell, int nx, int nz,
+                         void* stream) {
+  if (!grid_ok(nx, nz) || !lam || !hy_dens_cell) return PMW_EINVAL;
+  Grid g(nx, nz);
+  k_adj_halo_z<<<(unsigned)((nx + 4 + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      lam, g, hy_dens_cell);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
 
 int pmw_adjoint_stage(int direction, const double* state_forcing, const double* lam_in,
                       double scale, const double* acc1, const double* acc2,

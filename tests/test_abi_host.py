@@ -72,6 +72,10 @@ def test_argument_errors_return_codes_without_launch(lib):
                             1, 10, 1.0, 0.1, -1.0, None) == -1
     assert lib.pmw_set_bc_x(FAKE, 0, 10, 0, None, None, None, None) == -1
     assert lib.pmw_set_bc_z(None, FAKE, 10, 10, None) == -1
+    assert lib.pmw_adjoint_set_bc_x(FAKE, 1, 10, None) == -1
+    assert lib.pmw_adjoint_set_bc_x(None, 10, 10, None) == -1
+    assert lib.pmw_adjoint_set_bc_z(FAKE, None, 10, 10, None) == -1
+    assert lib.pmw_adjoint_set_bc_z(FAKE, FAKE, 10, 1, None) == -1
     # bad direction / bc enum
     assert lib.pmw_rk_stage(2, FAKE, P2, P3, FAKE, FAKE, FAKE, FAKE, FAKE, None, 0, None,
                             10, 10, 1.0, 0.1, -1.0, None) == -1

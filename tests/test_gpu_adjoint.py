@@ -68,6 +68,68 @@ def test_stage_adjoint_matches_autograd(direction, ic, nx, nz):
         assert rel_l2(got[v], ref[v]) <= 1e-12, f"var {v}: {rel_l2(got[v], ref[v]):.3e}"
 
 
+@pytest.mark.parametrize("nx,nz", [(13, 7), (2, 2), (3, 5)])
+def test_halo_adjoints_satisfy_dot_product_identity(nx, nz):
+    """<lam, H(S)> = <H^T lam, S> for the oracle's own halo functions H
+    (set_bc_x periodic, set_bc_z) and the device transposes, on random S and
+    lam over the whole buffer (halos, corners and sources shared for nx = 2,
+    3 included)."""
+    from pyminiweather_amd.adjoint import set_bc_x_adjoint, set_bc_z_adjoint
+    p = O.make_params(nx, nz, "thermal", cfl=1.5, max_speed=450.0)
+    f = _fields(p)
+    fnp = {k: v.cpu().numpy() for k, v in _fdict(f).items()}
+    rng = np.random.default_rng(nx * 100 + nz)
+    for halo, adj in ((O.halo_x, set_bc_x_adjoint), (O.halo_z, set_bc_z_adjoint)):
+        s = rng.standard_normal(f.shape)
+        lam = rng.standard_normal(f.shape)
+        hs = s.copy()
+        halo(p, fnp, hs)
+        lam_d = torch.from_numpy(lam).cuda()
+        adj(p, f, lam_d)
+        ht_lam = lam_d.cpu().numpy()
+        lhs = float(np.sum(lam * hs))
+        rhs = float(np.sum(ht_lam * s))
+        assert abs(lhs - rhs) <= 1e-13 * (np.sum(np.abs(lam * hs)) + np.sum(np.abs(ht_lam * s)))
+        # the transpose of an overwrite leaves nothing on the overwritten cells
+        written = hs != s
+        assert np.all(ht_lam[written] == 0.0)
+
+
+@pytest.mark.parametrize("ic", ["collision", "gravity"])
+def test_per_function_adjoints_match_autograd(ic):
+    """compute_tendencies_x/z_adjoint and discrete_step_adjoint against torch
+    autograd of the restated tendencies and stage."""
+    from oracle import torch_ref as R
+    from pyminiweather_amd.adjoint import (compute_tendencies_x_adjoint,
+                                           compute_tendencies_z_adjoint,
+                                           discrete_step_adjoint)
+    from pyminiweather_amd.ics import Directions
+    from pyminiweather_amd.solve import hv_coeff
+    p = O.make_params(36, 17, ic, cfl=1.5, max_speed=450.0)
+    f = _fields(p)
+    u = _perturbed_interior(f, 31).requires_grad_(True)
+    g = torch.Generator(device="cuda").manual_seed(37)
+    lam_t = torch.randn(u.shape, dtype=torch.float64, device="cuda", generator=g)
+    fs = torch.zeros(f.shape, dtype=torch.float64, device="cuda")
+    fs[:, 2:-2, 2:-2] = u.detach()
+    for d, tfun, adj in ((0, R.tend_x, compute_tendencies_x_adjoint),
+                         (1, R.tend_z, compute_tendencies_z_adjoint)):
+        dd = p["dx"] if d == 0 else p["dz"]
+        (ref,) = torch.autograd.grad((tfun(u, _fdict(f), dd, hv_coeff(dd, p["dt"])) * lam_t).sum(),
+                                     u)
+        got = adj(p, f, fs, lam_t)
+        assert torch.all(got[:, :2] == 0) and torch.all(got[:, :, :2] == 0)
+        r = rel_l2(got[:, 2:-2, 2:-2].cpu().numpy(), ref.cpu().numpy())
+        assert r <= 1e-12, f"direction {d}: {r:.3e}"
+        lam_out = torch.zeros_like(fs)
+        lam_out[:, 2:-2, 2:-2] = lam_t
+        li, lf = discrete_step_adjoint(p, f, fs, lam_out, 0.3 * p["dt"],
+                                       Directions.X if d == 0 else Directions.Z)
+        assert torch.equal(li[:, 2:-2, 2:-2], lam_t)
+        r = rel_l2(lf[:, 2:-2, 2:-2].cpu().numpy(), (0.3 * p["dt"] * ref).cpu().numpy())
+        assert r <= 1e-12, f"stage direction {d}: {r:.3e}"
+
+
 @pytest.mark.parametrize("ic,nx,nz,steps", [("thermal", 40, 20, 6), ("gravity", 24, 12, 5)])
 def test_gradient_matches_torch_autograd(ic, nx, nz, steps):
     """dL/dtheta' of L = 1/2 ||(rho*theta)'(T) - target||^2 over several full
