@@ -310,6 +310,11 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-members", type=int, default=64)
     ap.add_argument("--assim-iters", type=int, default=20)
+    ap.add_argument("--adjoint-steps", type=int, default=400,
+                    help="timesteps of the adjoint legs (400 = the workload; fewer only to "
+                         "profile the kernels)")
+    ap.add_argument("--fd-cells", type=int, default=8,
+                    help="cells of the single adjoint's finite-difference check")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 512 MiB device copies (scales the rocprof "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -430,8 +435,8 @@ def main():
         out["batched"] = bench_ensemble(args.ensemble_steps, args.ensemble_loop_steps,
                                         not args.no_autotune)
     if not args.no_adjoint:
-        out["adjoint"] = bench_adjoint()
-        out["batched_adjoint"] = bench_batched_adjoint(args.adjoint_members,
+        out["adjoint"] = bench_adjoint(args.adjoint_steps, args.fd_cells)
+        out["batched_adjoint"] = bench_batched_adjoint(args.adjoint_members, args.adjoint_steps,
                                                        iters=args.assim_iters)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(nx, nz)
