@@ -1133,7 +1133,7 @@ int pmw_ensemble_evolve(const pmw_member* members, int nmembers, const int* tuni
   if (kernel_ms) {
     kernel_ms[0] = kernel_ms[1] = 0.0;
     if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess) {
-      if (ev[0]) hipEventDestroy(ev[0]);
+      if (ev[0]) (void)hipEventDestroy(ev[0]);
       return PMW_EINVAL;
     }
   }
@@ -1141,21 +1141,25 @@ int pmw_ensemble_evolve(const pmw_member* members, int nmembers, const int* tuni
     const int first = *reverse_direction ? 0 : 1;  // 0 = x, 1 = z
     for (int pass = 0; pass < 2 && rc == PMW_OK; ++pass) {
       const int dir = pass == 0 ? first : 1 - first;
-      if (kernel_ms) hipEventRecord(ev[0], st);
+      if (kernel_ms) {
+        const hipError_t e = hipEventRecord(ev[0], st);
+        if (e != hipSuccess) { rc = (int)e; break; }
+      }
       rc = launch_ens(dir, x_nt, z_seg, d, nmembers, pass, plan, st);
       if (kernel_ms && rc == PMW_OK) {
-        hipEventRecord(ev[1], st);
+        hipError_t e = hipEventRecord(ev[1], st);
         float ms = 0.f;
-        if (hipEventSynchronize(ev[1]) == hipSuccess &&
-            hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess)
-          kernel_ms[dir] += ms;
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        if (e != hipSuccess) { rc = (int)e; break; }
+        kernel_ms[dir] += ms;
       }
     }
     if (rc == PMW_OK) *reverse_direction = !*reverse_direction;
   }
-  if (kernel_ms) {
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
+  if (kernel_ms) {  // measurement only: a failed destroy changes nothing
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
   }
   return rc;
 }

@@ -763,7 +763,7 @@ int pmw_evolve_ex(double* state, double* state_tmp, double* state_work,
       kernel_ms[dirs[j]] += ms;
     }
   }
-  for (int j = 0; j < made; ++j) hipEventDestroy(ev[j]);
+  for (int j = 0; j < made; ++j) (void)hipEventDestroy(ev[j]);  // cleanup only
   delete[] ev;
   delete[] dirs;
   return rc;
