@@ -8,10 +8,13 @@
 //    face flux into LDS, then one cell's tendency and RK update. Blocks whose
 //    window covers a halo column also store the wrapped value there, which is
 //    exactly set_bc_x.
-//  * z: each lane owns one interior column and marches up a segment of
-//    kZSeg rows with a 4-row register window, so every forcing value is
-//    loaded once per segment; the wall halos are synthesised in registers
-//    from the edge rows. An extra row of blocks writes the z halos of the
+//  * z: a 256-thread block holds a tile of 8 columns x 32 rows of the
+//    forcing state in LDS (the wall halo rows synthesised from the edge rows),
+//    computes the 29 face fluxes of its 28 output rows into LDS, then each
+//    cell's tendency and RK update; the 4 extra rows per 28 are all the
+//    forcing that is read twice (measured against 8x64, 8x48, 4x32, 16x32,
+//    16x64, 4x128 and 8x128 tiles, and a column-marching kernel: the small
+//    tile was fastest). Extra blocks after the tiles write the z halos of the
 //    forcing buffer (set_bc_z) concurrently; no compute block reads them.
 // No kernel reads a location another block of the same launch writes, which
 // is why `out` may not alias `forcing`.
@@ -24,9 +27,9 @@ namespace {
 
 constexpr int kXThreads = 256;
 constexpr int kXCells = kXThreads - 4;  // cells per x tile
-constexpr int kZLanes = 64;             // columns per z block
-constexpr int kZWaves = 4;              // z segments per block
-constexpr int kZSeg = 8;                // rows per z segment
+constexpr int kZtCols = 8;              // z tile columns
+constexpr int kZtRows = 32;             // z tile rows loaded (28 output rows)
+constexpr int kNumXcd = 8;              // XCDs of an MI355X
 
 inline bool bad_ptr(const void* p) { return p == nullptr; }
 
@@ -292,16 +295,25 @@ __device__ __forceinline__ void load_z_row(const double* f, const Grid& g,
     q[v] = wall_halo(v, f[g.at(v, edge, pcol)], hy[edge], hy[prow]);
 }
 
+// Tiles are ordered so that each XCD runs a contiguous range of them (tiles
+// next to each other share cache lines in that XCD's L2).
+__device__ __forceinline__ int zt_logical(int b, int per_xcd) {
+  return (b % kNumXcd) * per_xcd + b / kNumXcd;
+}
+
 template <int MODE>
-__global__ __launch_bounds__(kZLanes* kZWaves) void k_stage_z(
+__global__ __launch_bounds__(kZtCols* kZtRows) void k_stage_z(
     const double* init, double* forcing, double* out, double* __restrict__ tend,
     const double* __restrict__ hy_cell, const double* __restrict__ hyr,
     const double* __restrict__ hyt, const double* __restrict__ hyp,
-    const double* __restrict__ src_w, Grid g, double dz, double dt, double hv) {
-  const int nx = g.nx, nz = g.nz;
-  if (blockIdx.y == gridDim.y - 1) {
-    // halo writer row of blocks: set_bc_z on the forcing buffer
-    const int c = blockIdx.x * (kZLanes * kZWaves) + threadIdx.y * kZLanes + threadIdx.x;
+    const double* __restrict__ src_w, Grid g, double dz, double dt, double hv, int ncolgrp,
+    int ntiles, int per_xcd) {
+  constexpr int CW = kZtCols, NR = kZtRows, NT = CW * NR, C = NR - 4;
+  const int nx = g.nx, nz = g.nz, t = threadIdx.x;
+  const int tiles_padded = per_xcd * kNumXcd;
+  if ((int)blockIdx.x >= tiles_padded) {
+    // halo writer blocks: set_bc_z on the forcing buffer
+    const int c = ((int)blockIdx.x - tiles_padded) * NT + t;
     if (c >= g.ld) return;
     const int lo = kHs, hi = nz + kHs - 1;
 #pragma unroll
@@ -315,48 +327,49 @@ __global__ __launch_bounds__(kZLanes* kZWaves) void k_stage_z(
     }
     return;
   }
-  const int i = blockIdx.x * kZLanes + threadIdx.x;  // interior column
-  const int k0 = (blockIdx.y * kZWaves + threadIdx.y) * kZSeg;
-  if (i >= nx || k0 >= nz) return;
-  const int k1 = min(k0 + kZSeg, nz);
+  const int tile = zt_logical(blockIdx.x, per_xcd);
+  if (tile >= ntiles) return;  // whole block, before any barrier
+  __shared__ double s[kNv][NT];
+  __shared__ double fl[kNv][NT];
+  const int col = t % CW, j = t / CW;
+  const int i = (tile % ncolgrp) * CW + col;
+  const int k = (tile / ncolgrp) * C - 2 + j;  // this slot's row, face and cell
+  const bool colok = i < nx;
   const int pcol = i + kHs;
-  double w0[4], w1[4], w2[4], w3[4];
-  load_z_row(forcing, g, hy_cell, k0 - 2, pcol, w0);
-  load_z_row(forcing, g, hy_cell, k0 - 1, pcol, w1);
-  load_z_row(forcing, g, hy_cell, k0, pcol, w2);
-  double Gp[4];
-  for (int k = k0; k <= k1; ++k) {  // face k uses interior rows k-2..k+1
-    load_z_row(forcing, g, hy_cell, k + 1, pcol, w3);
+  double q[4] = {0.0, 0.0, 0.0, 0.0};
+  if (colok && k >= -2 && k <= nz + 1) load_z_row(forcing, g, hy_cell, k, pcol, q);
+#pragma unroll
+  for (int v = 0; v < kNv; ++v) s[v][t] = q[v];
+  __syncthreads();
+  // face k (between rows k-1 and k) from rows k-2..k+1 = tile rows j-2..j+1
+  if (colok && j >= 2 && j <= NR - 2 && k >= 0 && k <= nz) {
     double a[4], b[4], G[4];
 #pragma unroll
-    for (int v = 0; v < kNv; ++v) recon4(w0[v], w1[v], w2[v], w3[v], a[v], b[v]);
+    for (int v = 0; v < kNv; ++v)
+      recon4(s[v][t - 2 * CW], s[v][t - CW], s[v][t], s[v][t + CW], a[v], b[v]);
     flux_z_face(a, b, hyr[k], hyt[k], hyp[k], hv, k == 0 || k == nz, G);
-    if (k > k0) {
-      // cell k-1; its forcing rho' is interior row k-1 = window row 1
-      double td[4];
 #pragma unroll
-      for (int v = 0; v < kNv; ++v) td[v] = tend_of(Gp[v], G[v], dz, 1.0 / dz);
-      td[2] = td[2] - w1[0] * kGrav;
-      const long long cell = (long long)(k - 1) * nx + i;
-      if (src_w) td[2] = td[2] + src_w[cell];
-      if (MODE == 0) {
+    for (int v = 0; v < kNv; ++v) fl[v][t] = G[v];
+  }
+  __syncthreads();
+  // cell k from faces k and k+1 (tile rows 2..NR-3: the C output rows)
+  if (colok && j >= 2 && j <= NR - 3 && k >= 0 && k < nz) {
+    double td[4];
 #pragma unroll
-        for (int v = 0; v < kNv; ++v) {
-          const long long o = g.at(v, k - 1 + kHs, pcol);
-          out[o] = init[o] + dt * td[v];
-        }
-      } else {
-        const long long pt = (long long)nz * nx;
+    for (int v = 0; v < kNv; ++v) td[v] = tend_of(fl[v][t], fl[v][t + CW], dz, 1.0 / dz);
+    td[2] = td[2] - s[0][t] * kGrav;
+    const long long cell = (long long)k * nx + i;
+    if (src_w) td[2] = td[2] + src_w[cell];
+    if (MODE == 0) {
 #pragma unroll
-        for (int v = 0; v < kNv; ++v) tend[v * pt + cell] = td[v];
+      for (int v = 0; v < kNv; ++v) {
+        const long long o = g.at(v, k + kHs, pcol);
+        out[o] = init[o] + dt * td[v];
       }
-    }
+    } else {
+      const long long pt = (long long)nz * nx;
 #pragma unroll
-    for (int v = 0; v < kNv; ++v) {
-      Gp[v] = G[v];
-      w0[v] = w1[v];
-      w1[v] = w2[v];
-      w2[v] = w3[v];
+      for (int v = 0; v < kNv; ++v) tend[v * pt + cell] = td[v];
     }
   }
 }
@@ -388,16 +401,17 @@ int launch_stage_z(int mode, const double* init, double* forcing, double* out,
                    double* tend, const double* hy_cell, const double* hyr,
                    const double* hyt, const double* hyp, const double* src_w,
                    Grid g, double dz, double dt, double hv, hipStream_t st) {
-  const unsigned gx = blocks_for(g.nx, kZLanes);
-  const unsigned gy = blocks_for(g.nz, kZSeg * kZWaves) + 1;  // + halo writers
-  // the halo-writer row covers nx+4 columns with gx * 256 threads
-  dim3 grid(gx, gy), block(kZLanes, kZWaves);
+  constexpr int NT = kZtCols * kZtRows;
+  const int ncolgrp = (g.nx + kZtCols - 1) / kZtCols;
+  const int ntiles = ncolgrp * ((g.nz + kZtRows - 5) / (kZtRows - 4));
+  const int per = (ntiles + kNumXcd - 1) / kNumXcd;
+  const unsigned nblk = (unsigned)per * kNumXcd + blocks_for(g.ld, NT);  // + halo writers
   if (mode == 0)
-    k_stage_z<0><<<grid, block, 0, st>>>(init, forcing, out, tend, hy_cell, hyr,
-                                        hyt, hyp, src_w, g, dz, dt, hv);
+    k_stage_z<0><<<nblk, NT, 0, st>>>(init, forcing, out, tend, hy_cell, hyr, hyt, hyp,
+                                      src_w, g, dz, dt, hv, ncolgrp, ntiles, per);
   else
-    k_stage_z<1><<<grid, block, 0, st>>>(init, forcing, out, tend, hy_cell, hyr,
-                                        hyt, hyp, src_w, g, dz, dt, hv);
+    k_stage_z<1><<<nblk, NT, 0, st>>>(init, forcing, out, tend, hy_cell, hyr, hyt, hyp,
+                                      src_w, g, dz, dt, hv, ncolgrp, ntiles, per);
   PMW_CHECK_LAUNCH();
   return PMW_OK;
 }
