@@ -9,8 +9,10 @@ ctypes call into libpmw.so:
 * ``discrete_step`` (solve/step.py:21-82): one fused kernel per call
   (halo + reconstruction + flux + tendency + source + RK update); when
   ``state_out`` overlaps ``state_forcing`` (the reference's aliased stage 2)
-  it runs the per-function kernels instead, whose tendency is complete
-  before the update, exactly like the reference.
+  it runs the fused tendency kernel into ``fields.tend`` and then the
+  update, so the tendency is complete before the update, exactly like the
+  reference (``fused=False`` runs the per-function kernels, which also fill
+  the reference's intermediates).
 * ``evolve`` (solve/step.py:85-143): one launch per direction, each running
   the three RK stages of that direction in LDS tiles (per-stage launches
   for the injection BC, or when chosen by the tuning). The first direction
@@ -165,6 +167,14 @@ def discrete_step(params, fields, mesh, state_init, state_forcing, state_out, dt
                   ptr(fields.hy_pressure_int), _src(params, fields), bc_code(ic_type),
                   _inj(fields, ic_type), fields.nx, fields.nz, float(dx_or_dz), float(dt),
                   hv_coeff(dx_or_dz, params["dt"]), _st(fields))
+        return
+    if fused:  # out aliases forcing: the whole tendency first, then the update
+        if direction == Directions.X:
+            compute_tendencies_x(params, fields, state_forcing)
+        else:
+            compute_tendencies_z(params, fields, state_forcing)
+        _lib.call("pmw_rk_update", ptr(state_init), ptr(fields.tend), ptr(state_out),
+                  fields.nx, fields.nz, float(dt), _st(fields))
         return
     if direction == Directions.X:
         set_bc_x(params, fields, state_forcing, ic_type)

@@ -238,6 +238,14 @@ def test_fused_equals_unfused_bitwise(stage_case):
             outs.append((host(b), host(c)))
         np.testing.assert_array_equal(outs[0][0], outs[1][0])
         np.testing.assert_array_equal(outs[0][1], outs[1][1])
+        # the aliased stage (out is forcing): fused tendency + update vs the
+        # per-function kernels
+        outs = []
+        for fused in (True, False):
+            a, b = _t(g["s_init"]), _t(g["s_forc"])
+            discrete_step(p, f, mesh, a, b, b, 0.05, direction, fused=fused)
+            outs.append(host(b))
+        np.testing.assert_array_equal(outs[0], outs[1])
 
 
 def test_fused_tendencies_equal_unfused(stage_case):
