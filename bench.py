@@ -320,9 +320,9 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the x-slab halo-exchange design is described in DESIGN.md)",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (one GPU runs "
+              "the whole grid; a multi-GPU run would split it into x-slabs with a "
+              "6-column halo exchange per x direction)", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -1,7 +1,7 @@
 """HIP path against the oracle and the reference's golden vectors (GPU).
 
 Tolerance: relative L2 per variable over the interior <= 1e-12 after N steps
-Halo copies and reconstructions have no rounding
+(the forward's parity bar). Halo copies and reconstructions have no rounding
 freedom and are compared bit for bit; flux-bearing results differ from NumPy
 only through pow's last bit, so single-stage results are held to 1e-13.
 """
