@@ -327,7 +327,7 @@ def test_autograd_function_composes():
         assert rel_l2(ga[v].cpu().numpy(), gb[v].cpu().numpy()) <= 1e-11
 
 
-def _config3():
+def _thermal_400x200_misfit():
     """400x200 thermal, 400 RK3 steps; the target is the forward run from
     the thermal IC plus i.i.d. N(0, 0.1 K) theta' noise drawn with numpy
     default_rng(1). Returns (solver, theta0, target, loss, grad)."""
@@ -350,7 +350,7 @@ def _central_fd(solver, theta0, target, k, i, eps):
     return (solver.loss(theta0 + e, target) - solver.loss(theta0 - e, target)) / (2 * eps)
 
 
-def test_config3_gradient_matches_central_differences():
+def test_400x200_gradient_matches_central_differences():
     """The adjoint gradient w.r.t. the initial theta' (sqrt(N) = 20-step
     checkpoints) against central differences on 64 random cells.
 
@@ -365,7 +365,7 @@ def test_config3_gradient_matches_central_differences():
     truncation error of the central difference is still ~1e-10, the same
     per-cell bound holds and the 64 quotients agree with the adjoint to 1e-6
     relative in the L2 norm over the 64 cells."""
-    solver, theta0, target, loss, grad = _config3()
+    solver, theta0, target, loss, grad = _thermal_400x200_misfit()
     assert solver.every == 20
     assert loss > 0
     base = solver.loss(theta0, target)
@@ -394,7 +394,7 @@ def test_config3_gradient_matches_central_differences():
     print("; ".join(report))
 
 
-def test_config4_batched_assimilation_reduces_every_loss():
+def test_64_member_assimilation_reduces_every_loss():
     """64 members on 400x200, 400 steps: member m's target is the forward
     run from the thermal IC plus N(0, 0.1 K) theta' noise drawn with numpy
     default_rng(2 + m); starting from the thermal IC, 20 steepest-descent
