@@ -75,7 +75,7 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(nx, nz, steps=1):
+def cpu_baseline(nx, nz, steps=3):
     """The oracle's SciPy-convolve variant (the reference's own arithmetic
     path) timed for `steps` timesteps of the same workload on 1 host core."""
     import oracle as O
