@@ -19,6 +19,9 @@ import numpy as np
 
 def get_logger(filename=None):
     logger = logging.getLogger("pyminiweather")
+    for h in list(logger.handlers):  # a second run in one process logs to its own target only
+        logger.removeHandler(h)
+        h.close()
     handler = logging.StreamHandler() if filename is None else logging.FileHandler(
         filename, mode="w")
     handler.setFormatter(logging.Formatter("%(message)s"))

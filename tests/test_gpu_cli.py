@@ -10,20 +10,29 @@ from conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def test_cli_outputs_and_log_match_oracle(tmp_path):
-    """collision 40x20, 5 steps, output every 2 steps: the state and
-    solution-variable files hold the snapshots taken before steps 1 and 3
-    (after 1 and 3 steps) in the reference's savetxt layout, and the log
-    reports the oracle's start/end mass and energy and max(rho*theta)."""
+@pytest.mark.parametrize("ic,warmups", [("collision", 0), ("injection", 0),
+                                        ("density-current", 2)])
+def test_cli_outputs_and_log_match_oracle(tmp_path, ic, warmups):
+    """40x20, 5 steps (after `warmups` warm-up steps), output every 2 steps:
+    the state and solution-variable files hold the snapshots taken before
+    steps 1 and 3 in the reference's savetxt layout, and the log reports the
+    oracle's start (before the warm-ups) and end mass and energy and
+    max(rho*theta). The injection IC runs the per-stage kernels."""
+    import pyminiweather_amd.solve as S
     from pyminiweather_amd.__main__ import main
     out = tmp_path / "run.txt"
     log = tmp_path / "log.txt"
+    S.reverse_direction = False
     rc = main(["--nx", "40", "--nz", "20", "--nsteps", "5", "--output-freq", "2",
-               "--ic-type", "collision", "--app-filename", str(out), "--app-log-file", str(log)])
+               "--nwarmups", str(warmups), "--ic-type", ic, "--app-filename", str(out),
+               "--app-log-file", str(log)])
+    S.reverse_direction = False
     assert rc == 0
-    p = O.make_params(40, 20, "collision")
+    p = O.make_params(40, 20, ic)
     orc = O.OracleSolver(p)
     m0, e0 = orc.stats()
+    for _ in range(warmups):
+        orc.timestep()
     snaps, svars, maxes = [], [], []
     for istep in range(5):
         if (istep + 1) % 2 == 0:
@@ -39,10 +48,30 @@ def test_cli_outputs_and_log_match_oracle(tmp_path):
     gsv = sv.reshape(2, 4, 20, 40)
     for k in range(2):
         for v in range(4):
+            ref, refs = snaps[k][v], svars[k][v]
+            print(f"{ic} snapshot {k} var {v}: state rel {rel_l2(got[k, v], ref):.3e} "
+                  f"max|ref| {np.max(np.abs(ref)):.3e} max|diff| "
+                  f"{np.max(np.abs(got[k, v] - ref)):.3e}; svars rel {rel_l2(gsv[k, v], refs):.3e} "
+                  f"max|ref| {np.max(np.abs(refs)):.3e} max|diff| "
+                  f"{np.max(np.abs(gsv[k, v] - refs)):.3e}", flush=True)
+    # theta' = (rho*theta' + hyt)/(hyr + rho') - hyt/hyr subtracts two values
+    # near the full theta (~300 K), so it carries ~1 ulp of 300 K whatever its
+    # inputs; where theta' itself is small (the injection inflow, < 0.2 K) that
+    # floor exceeds 1e-12 relative, and theta' is held to 4 ulp of the full
+    # theta instead (the states it is computed from agree to ~1e-14)
+    theta_full = np.max(orc.f["hy_dens_theta_cell"] / orc.f["hy_dens_cell"])
+    for k in range(2):
+        for v in range(4):
             ref = snaps[k][v]
             assert rel_l2(got[k, v], ref) <= 1e-12 if np.any(ref) else np.max(np.abs(got[k, v])) < 1e-12
             refs = svars[k][v]
-            assert rel_l2(gsv[k, v], refs) <= 1e-12 if np.any(refs) else np.max(np.abs(gsv[k, v])) < 1e-12
+            if not np.any(refs):
+                assert np.max(np.abs(gsv[k, v])) < 1e-12
+            elif v == 3:
+                assert (rel_l2(gsv[k, v], refs) <= 1e-12 or
+                        np.max(np.abs(gsv[k, v] - refs)) <= 4 * np.spacing(theta_full))
+            else:
+                assert rel_l2(gsv[k, v], refs) <= 1e-12
     text = log.read_text()
     nums = lambda pat: [float(x) for x in re.search(pat, text).groups()]
     gm0, ge0 = nums(r"Start: total_mass, total_energy: (\S+), (\S+)")
