@@ -130,7 +130,9 @@ def test_per_function_adjoints_match_autograd(ic):
         assert r <= 1e-12, f"stage direction {d}: {r:.3e}"
 
 
-@pytest.mark.parametrize("ic,nx,nz,steps", [("thermal", 40, 20, 6), ("gravity", 24, 12, 5)])
+@pytest.mark.parametrize("ic,nx,nz,steps", [("thermal", 40, 20, 6), ("gravity", 24, 12, 5),
+                                             ("density-current", 32, 16, 4),
+                                             ("collision", 30, 14, 3)])
 def test_gradient_matches_torch_autograd(ic, nx, nz, steps):
     """dL/dtheta' of L = 1/2 ||(rho*theta)'(T) - target||^2 over several full
     timesteps (both direction orders, sqrt(N) checkpoints) equals FP64 torch
