@@ -171,3 +171,25 @@ def test_mixed_ensemble_draws_are_seeded_and_in_range():
         assert 1.0 <= amp <= 5.0 and 6e3 <= x0 <= 14e3 and 1.5e3 <= z0 <= 3e3
         assert 1.5e3 <= xr <= 2.5e3 and xr == zr
         assert p["dt"] == min(p["dx"], p["dz"]) * 1.5 / 450.0
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """Without libpmw.so every entry point raises (no CPU fallback): checked
+    in a fresh interpreter whose library path points at a missing file."""
+    import subprocess
+    import sys
+    code = ("import pyminiweather_amd._lib as L\n"
+            f"L.LIB_PATH = {str(tmp_path / 'missing.so')!r}\n"
+            "import pyminiweather_amd.solve as S\n"
+            "try:\n"
+            "    S.hv_coeff(1.0, 1.0)\n"
+            "    L.call('pmw_abi_version')\n"
+            "except RuntimeError as e:\n"
+            "    print('raised:', e)\n"
+            "    raise SystemExit(0)\n"
+            "raise SystemExit(3)\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "no CPU fallback" in r.stdout and "missing.so" in r.stdout
