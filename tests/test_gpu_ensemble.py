@@ -143,3 +143,43 @@ def test_ensemble_rejects_bad_tuning_and_injection():
         _ensemble(tuning=(1, 256, 16))
     with pytest.raises(ValueError):
         Ensemble([member_params(20, 10, "injection")])
+
+
+def test_full_256_member_ensemble():
+    """The seeded 256-member mixed-size ensemble at full size, 6 steps: every
+    member conserves mass to 1e-13 relative with finite energy, and the
+    first member of each grid size equals its own single-grid solve bit for
+    bit."""
+    import pyminiweather_amd.solve as S
+    from pyminiweather_amd.data import initialize_fields
+    from pyminiweather_amd.ensemble import GRID_CHOICES, Ensemble, mixed_thermal_members
+    from pyminiweather_amd.ics import init
+    from pyminiweather_amd.mesh import MeshData
+    tuning = (0, 256, 8128)
+    params, bubbles = mixed_thermal_members(256, seed=0)
+    ens = Ensemble(params, bubbles, tuning=tuning)
+    assert len(ens) == 256 and {(p["nx"], p["nz"]) for p in params} == set(GRID_CHOICES)
+    before = ens.stats()
+    ens.evolve(6)
+    after = ens.stats()
+    for q, ((m0, e0), (m1, e1)) in enumerate(zip(before, after)):
+        assert abs(m1 - m0) <= 1e-13 * abs(m0), f"member {q}: mass {m0!r} -> {m1!r}"
+        assert np.isfinite(e1) and abs(e1 - e0) <= 1e-3 * abs(e0), f"member {q}"
+    firsts = {}
+    for q, p in enumerate(params):
+        firsts.setdefault((p["nx"], p["nz"]), q)
+    for (nx, nz), q in sorted(firsts.items()):
+        p = params[q]
+        f = initialize_fields(p)
+        mesh = MeshData(p)
+        init(f, p, mesh, bubbles=[bubbles[q]])
+        S.reverse_direction = False
+        S.set_tuning(nx, nz, tuning)
+        try:
+            S.run(p, f, mesh, 6)
+        finally:
+            S._tuning_table.pop((nx, nz), None)
+            S.reverse_direction = False
+        print(f"member {q} ({nx}x{nz}) against its single solve", flush=True)
+        np.testing.assert_array_equal(_host(ens.fields[q].state), _host(f.state))
+        np.testing.assert_array_equal(_host(ens.fields[q].state_tmp), _host(f.state_tmp))
