@@ -240,7 +240,8 @@ int pmw_ensemble_prepare(const pmw_member* members, int nmembers, const int* tun
  * `nsteps` timesteps, each one launch per direction over the concatenated
  * tile lists of all members (the three RK stages fused as in
  * pmw_evolve_ex). `members`
- * and `tuning` must be what was prepared. Only kernel launches are issued,
+ * and `tuning` must be what was prepared (with others the states are wrong,
+ * but every tile stays inside its member's rows). Only kernel launches are issued,
  * so a call can be captured in a HIP graph. Each member ends bit-identical
  * to pmw_evolve on that member alone. kernel_ms as in pmw_evolve_ex (each
  * launch is synchronised when given). */

@@ -831,7 +831,9 @@ __global__ __launch_bounds__(NT) void k_ens_x(const MemberDev* __restrict__ m, i
   const MemberDev& md = m[member_of<0>(m, n, item)];
   const DirArgs a = member_args(md, 0, pass);
   const int local = item - md.x_start;
-  dir_x_tile<NT>(a, Grid(md.nx, md.nz), local / md.x_ntiles, (local % md.x_ntiles) * C, sm);
+  const int row = local / md.x_ntiles;
+  if (row >= md.nz) return;  // a plan that is not the prepared one: stay in bounds
+  dir_x_tile<NT>(a, Grid(md.nx, md.nz), row, (local % md.x_ntiles) * C, sm);
 }
 
 template <int L>
@@ -843,6 +845,7 @@ __global__ __launch_bounds__(192) void k_ens_z(const MemberDev* __restrict__ m, 
   const MemberDev& md = m[member_of<1>(m, n, item)];
   const DirArgs a = member_args(md, 1, pass);
   const int local = item - md.z_start;
+  if ((long long)(local / md.z_ncol) * L >= md.nz) return;  // as in k_ens_x
   dir_z_block<L>(a, Grid(md.nx, md.nz), local % md.z_ncol, local / md.z_ncol, ring);
 }
 
@@ -856,6 +859,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
   const MemberDev& md = m[member_of<1>(m, n, item)];
   const DirArgs a = member_args(md, 1, pass);
   const int local = item - md.z_start;
+  if ((long long)(local / md.z_ncol) * C >= md.nz) return;  // as in k_ens_x
   dir_zt_tile<CW, NT>(a, Grid(md.nx, md.nz), (local % md.z_ncol) * CW,
                       (local / md.z_ncol) * C, sm, local == 0);
 }
@@ -869,6 +873,7 @@ __global__ __launch_bounds__(CW * NR / 2) void k_ens_zt2(const MemberDev* __rest
   const MemberDev& md = m[member_of<1>(m, n, item)];
   const DirArgs a = member_args(md, 1, pass);
   const int local = item - md.z_start;
+  if ((long long)(local / md.z_ncol) * (NR - 12) >= md.nz) return;  // as in k_ens_x
   dir_zt2_tile<CW, NR>(a, Grid(md.nx, md.nz), (local % md.z_ncol) * CW,
                        (local / md.z_ncol) * (NR - 12), sm, local == 0);
 }
