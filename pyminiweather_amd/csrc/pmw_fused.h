@@ -21,6 +21,9 @@ struct DirArgs {
   double dt1, dt2, dt3;    // dt/3, dt/2, dt
   double hv;               // -hv_beta * d / (16 * dt_full)
   gdouble* s1 = nullptr;   // optional: receives the stage-1 interior
+  bool tmp_interior = true;  // false: tmp gets only its halo refresh (its
+                             // interior is dead when the next direction of
+                             // the same step rewrites every interior cell)
   long long mstride = 0;   // batched launches: elements between members of
                            // in / st / out / tmp / s1 (member = blockIdx.y)
 };

@@ -140,7 +140,7 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
 #pragma unroll
     for (int v = 0; v < kNv; ++v) s1[v][t] = q[v];
     // state_tmp: own cells, plus its x halos (set_bc_x of stage 3's forcing)
-    const bool own = cell >= i0 && cell < i0 + C && cell < nx;
+    const bool own = a.tmp_interior && cell >= i0 && cell < i0 + C && cell < nx;
     const bool halo = cell == -2 || cell == -1 || cell == nx || cell == nx + 1;
     if (own || halo) {
 #pragma unroll
@@ -344,7 +344,7 @@ __device__ __forceinline__ void z_block_step(double w[4][4], double F[4], const 
       }
       if (stage == 0 && a.s1 && c >= k0 && c < k1) store_row(a.s1, g, c + kHs, pc, res);
       if (stage == 1) {
-        if (c >= k0 && c < k1) store_row(a.tmp, g, c + kHs, pc, res);
+        if (a.tmp_interior && c >= k0 && c < k1) store_row(a.tmp, g, c + kHs, pc, res);
         if (c == 0 && k0 == 0) {  // set_bc_z of stage 3 into tmp
           double h[4];
           wall_row(res, hy, kHs, 0, h);
@@ -500,6 +500,7 @@ __device__ __forceinline__ void dir_zt_tile(const DirArgs& a, const Grid& g, int
   const bool wlo = k == -2 || k == -1, whi = k == nz || k == nz + 1;
   const bool own = colok && k >= r0 && k < r0 + C && k < nz;
   const bool own_lo = r0 == 0, own_hi = r0 <= nz - 1 && nz - 1 < r0 + C;
+  gdouble* const tmp_own = a.tmp_interior ? a.tmp : nullptr;
   // slot of the edge row a wall slot is built from
   const int edge_slot = wlo ? t - k * CW : t - (k - (nz - 1)) * CW;
   {
@@ -570,7 +571,7 @@ __device__ __forceinline__ void dir_zt_tile(const DirArgs& a, const Grid& g, int
         for (int v = 0; v < kNv; ++v) s1[v][t] = q[v];
       }
       if (own) {
-        gdouble* dst = stage == 0 ? a.s1 : stage == 1 ? a.tmp : a.out;
+        gdouble* dst = stage == 0 ? a.s1 : stage == 1 ? tmp_own : a.out;
         if (dst) {
 #pragma unroll
           for (int v = 0; v < kNv; ++v) dst[g.at(v, k + kHs, pc)] = q[v];
@@ -635,6 +636,7 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
   const gdouble* hy = a.hy_cell;
   const int plo = kHs, phi = nz + kHs - 1;
   const bool own_lo = r0 == 0, own_hi = r0 <= nz - 1 && nz - 1 < r0 + C;
+  gdouble* const tmp_own = a.tmp_interior ? a.tmp : nullptr;
   int jj[2], kk[2], sl[2], es[2];
   bool wl[2], wh[2], own[2];
   double S[2][4], src[2];
@@ -724,7 +726,7 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
           for (int v = 0; v < kNv; ++v) s[v][u] = q[v];  // in place: faces are done
         }
         if (own[c]) {
-          gdouble* dst = stage == 0 ? a.s1 : stage == 1 ? a.tmp : a.out;
+          gdouble* dst = stage == 0 ? a.s1 : stage == 1 ? tmp_own : a.out;
           if (dst) {
 #pragma unroll
             for (int v = 0; v < kNv; ++v) dst[g.at(v, k + kHs, pc)] = q[v];
@@ -788,6 +790,7 @@ __device__ __forceinline__ DirArgs member_args(const MemberDev& m, int dir, int 
   a.out = gptr(pass == 0 ? m.work : m.state);
   a.st = gptr(m.state);
   a.tmp = gptr(m.tmp);
+  a.tmp_interior = pass == 1;  // the second direction rewrites all of it
   a.hy_cell = gptr(m.hyc);
   a.hyt_cell = gptr(m.hytc);
   a.hyr_int = gptr(m.hyri);

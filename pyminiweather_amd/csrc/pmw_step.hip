@@ -673,6 +673,7 @@ int evolve_impl(double* state, double* state_tmp, double* state_work,
         a.out = gptr(pass == 0 ? state_work : state);
         a.st = gptr(state);
         a.tmp = gptr(state_tmp);
+        a.tmp_interior = pass == 1;  // the second direction rewrites all of it
         a.hy_cell = gptr(hy_dens_cell);
         a.hyt_cell = gptr(hy_dens_theta_cell);
         a.hyr_int = gptr(hy_dens_int);
