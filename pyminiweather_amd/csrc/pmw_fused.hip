@@ -784,13 +784,14 @@ struct MemberDev {
   int x_ntiles, z_ncol;  // x tiles per row, z column groups
 };
 
-__device__ __forceinline__ DirArgs member_args(const MemberDev& m, int dir, int pass) {
+__device__ __forceinline__ DirArgs member_args(const MemberDev& m, int dir, int pass,
+                                               bool tmp_interior) {
   DirArgs a;
   a.in = gptr(pass == 0 ? m.state : m.work);
   a.out = gptr(pass == 0 ? m.work : m.state);
   a.st = gptr(m.state);
   a.tmp = gptr(m.tmp);
-  a.tmp_interior = pass == 1;  // the second direction rewrites all of it
+  a.tmp_interior = tmp_interior;
   a.hy_cell = gptr(m.hyc);
   a.hyt_cell = gptr(m.hytc);
   a.hyr_int = gptr(m.hyri);
@@ -823,7 +824,9 @@ __device__ __forceinline__ int member_of(const MemberDev* m, int n, int item) {
 // xcd_logical(b), so neighbouring tiles of a member share an XCD's L2, and
 // finds its member by a scalar binary search over the start offsets. (A
 // persistent form, each block looping over a chunk of items, kept ~70 more
-// VGPRs live and halved the occupancy.)
+// VGPRs live and halved the occupancy.) `pass`: bit 0 = direction of the
+// step (0: state -> work, 1: work -> state), bit 1 = store state_tmp's
+// interior (the last direction of the call only).
 template <int NT>
 __global__ __launch_bounds__(NT) void k_ens_x(const MemberDev* __restrict__ m, int n, int pass,
                                               int total, int per_xcd) {
@@ -832,7 +835,7 @@ __global__ __launch_bounds__(NT) void k_ens_x(const MemberDev* __restrict__ m, i
   const int item = xcd_logical(blockIdx.x, per_xcd);
   if (item >= total) return;  // whole block, before any barrier
   const MemberDev& md = m[member_of<0>(m, n, item)];
-  const DirArgs a = member_args(md, 0, pass);
+  const DirArgs a = member_args(md, 0, pass & 1, pass >> 1);
   const int local = item - md.x_start;
   const int row = local / md.x_ntiles;
   if (row >= md.nz) return;  // a plan that is not the prepared one: stay in bounds
@@ -846,7 +849,7 @@ __global__ __launch_bounds__(192) void k_ens_z(const MemberDev* __restrict__ m, 
   const int item = xcd_logical(blockIdx.x, per_xcd);
   if (item >= total) return;  // whole block, before any barrier
   const MemberDev& md = m[member_of<1>(m, n, item)];
-  const DirArgs a = member_args(md, 1, pass);
+  const DirArgs a = member_args(md, 1, pass & 1, pass >> 1);
   const int local = item - md.z_start;
   if ((long long)(local / md.z_ncol) * L >= md.nz) return;  // as in k_ens_x
   dir_z_block<L>(a, Grid(md.nx, md.nz), local % md.z_ncol, local / md.z_ncol, ring);
@@ -860,7 +863,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
   const int item = xcd_logical(blockIdx.x, per_xcd);
   if (item >= total) return;  // whole block, before any barrier
   const MemberDev& md = m[member_of<1>(m, n, item)];
-  const DirArgs a = member_args(md, 1, pass);
+  const DirArgs a = member_args(md, 1, pass & 1, pass >> 1);
   const int local = item - md.z_start;
   if ((long long)(local / md.z_ncol) * C >= md.nz) return;  // as in k_ens_x
   dir_zt_tile<CW, NT>(a, Grid(md.nx, md.nz), (local % md.z_ncol) * CW,
@@ -874,7 +877,7 @@ __global__ __launch_bounds__(CW * NR / 2) void k_ens_zt2(const MemberDev* __rest
   const int item = xcd_logical(blockIdx.x, per_xcd);
   if (item >= total) return;  // whole block, before any barrier
   const MemberDev& md = m[member_of<1>(m, n, item)];
-  const DirArgs a = member_args(md, 1, pass);
+  const DirArgs a = member_args(md, 1, pass & 1, pass >> 1);
   const int local = item - md.z_start;
   if ((long long)(local / md.z_ncol) * (NR - 12) >= md.nz) return;  // as in k_ens_x
   dir_zt2_tile<CW, NR>(a, Grid(md.nx, md.nz), (local % md.z_ncol) * CW,
@@ -1153,7 +1156,9 @@ int pmw_ensemble_evolve(const pmw_member* members, int nmembers, const int* tuni
         const hipError_t e = hipEventRecord(ev[0], st);
         if (e != hipSuccess) { rc = (int)e; break; }
       }
-      rc = launch_ens(dir, x_nt, z_seg, d, nmembers, pass, plan, st);
+      // state_tmp's interior: only the call's last direction keeps its stores
+      const int keep_tmp = pass == 1 && step == nsteps - 1;
+      rc = launch_ens(dir, x_nt, z_seg, d, nmembers, pass | keep_tmp << 1, plan, st);
       if (kernel_ms && rc == PMW_OK) {
         hipError_t e = hipEventRecord(ev[1], st);
         float ms = 0.f;

@@ -673,7 +673,9 @@ int evolve_impl(double* state, double* state_tmp, double* state_work,
         a.out = gptr(pass == 0 ? state_work : state);
         a.st = gptr(state);
         a.tmp = gptr(state_tmp);
-        a.tmp_interior = pass == 1;  // the second direction rewrites all of it
+        // state_tmp's interior is rewritten by every direction, and no launch
+        // reads it: only the last direction of the call keeps its stores
+        a.tmp_interior = pass == 1 && step == nsteps - 1;
         a.hy_cell = gptr(hy_dens_cell);
         a.hyt_cell = gptr(hy_dens_theta_cell);
         a.hyr_int = gptr(hy_dens_int);
