@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 # forcing + init and write out: 64 + 96 + 96 = 256 B, 512 B per step. One
 # launch of the fused kernels advances one direction (all three stages), so
 # its algorithmic figure is 256 B per cell; its actual traffic (S read once,
-# result and state_tmp written) is measured by the rocprof counters.
+# the result written) is measured by the rocprof counters.
 BYTES_PER_CELL_STEP = 512.0
 BYTES_PER_CELL_LAUNCH = BYTES_PER_CELL_STEP / 2.0
 # fwd+adjoint with checkpoint recompute, per cell-step: forward 512 + segment
@@ -92,14 +92,19 @@ def cpu_baseline(nx, nz, steps=3):
                       f"os.cpu_count()={os.cpu_count()}"}
 
 
+GRAPH_STEPS = 10  # timesteps per replay of the ensemble's HIP graph
+
+
 def bench_ensemble(nsteps, loop_steps, autotune):
     """Batched form: the 256-member mixed-size thermal ensemble (seed 0),
-    `nsteps` timesteps as HIP-graph replays of 2-step graphs, against looping
+    `nsteps` timesteps as replays of a 10-step HIP graph, against looping
     single-grid solves over the same members for `loop_steps` steps."""
     import torch
     import pyminiweather_amd.solve as S
     from pyminiweather_amd.ensemble import Ensemble, mixed_thermal_members
     from pyminiweather_amd.autotune import tune_ensemble, tune_grid
+    if nsteps < GRAPH_STEPS:
+        raise SystemExit(f"--ensemble-steps must be at least {GRAPH_STEPS}")
     params, bubbles = mixed_thermal_members(256, seed=0)
     ens = Ensemble(params, bubbles)
     if autotune:
@@ -112,15 +117,15 @@ def bench_ensemble(nsteps, loop_steps, autotune):
                 tune_grid(p, f)
     m0 = [m for m, _ in ens.stats()]
     ens.evolve(2)
-    graph = ens.capture(step_pairs=1)
+    graph = ens.capture(step_pairs=GRAPH_STEPS // 2)
     graph.replay()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(nsteps // 2):
+    for _ in range(nsteps // GRAPH_STEPS):
         graph.replay()
     torch.cuda.synchronize()
     t_batched = time.perf_counter() - t0
-    steps_done = 2 * (nsteps // 2)
+    steps_done = GRAPH_STEPS * (nsteps // GRAPH_STEPS)
     rate = ens.cells * steps_done / t_batched
     m1 = [m for m, _ in ens.stats()]
     drift = max(abs(b - a) / a for a, b in zip(m0, m1))
@@ -170,7 +175,8 @@ def bench_ensemble(nsteps, loop_steps, autotune):
             "speedup_vs_evolve_loop": rate / step_loop_rate,
             "step_roofline_frac": round(rate * BYTES_PER_CELL_STEP / 1e9 / HBM_PEAK_GBS, 4),
             "max_rel_mass_drift": drift,
-            "launch": "one launch per direction and step over the flattened tile list of all members, 2-step HIP graph",
+            "launch": ("one launch per direction and step over the flattened tile list of all "
+                       f"members, {GRAPH_STEPS}-step HIP graph"),
             "tuning": list(ens.tuning)}
 
 

@@ -117,17 +117,20 @@ def test_members_match_oracle():
             assert r <= tol, f"member {q} var {v}: {r:.3e} > {tol:.2e}"
 
 
-def test_graph_replays_match_eager():
-    """Each replay of a captured 2-step graph, with eager work and host syncs
-    between replays, matches an eagerly stepped twin bit for bit."""
+@pytest.mark.parametrize("pairs", [1, 3])
+def test_graph_replays_match_eager(pairs):
+    """Each replay of a captured 2- or 6-step graph, with eager work and host
+    syncs between replays, matches a twin stepped eagerly one step per call
+    bit for bit."""
     a = _ensemble()
     b = _ensemble()
-    g = a.capture(step_pairs=1)
+    g = a.capture(step_pairs=pairs)
     # capture records but does not run: states are unchanged
     np.testing.assert_array_equal(_host(a.fields[0].state), _host(b.fields[0].state))
     for rep in range(3):
         g.replay()
-        b.evolve(2)
+        for _ in range(2 * pairs):
+            b.evolve(1)
         _ = torch.ones(1000, device="cuda").sum().item()  # unrelated eager work
         torch.cuda.synchronize()
         for q in range(len(MEMBERS)):
