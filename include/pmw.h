@@ -183,8 +183,10 @@ int pmw_evolve(double* state, double* state_tmp, double* state_work,
  *   [0] mode: 0 = one fused launch per direction (all three RK stages,
  *       periodic x only; the injection BC always uses mode 1), 1 = one
  *       launch per RK stage (the pmw_rk_stage kernels);
- *   [1] fused x block size: 256, 448 or 512 threads (tile = that minus 12
- *       cells);
+ *   [1] fused x tiles: 256, 448 or 512 threads per block, one row per block
+ *       (tile = that minus 12 cells); 1256 / 1512 = dense tiles of 256 /
+ *       512 threads over the rows laid end to end (each row with its 6
+ *       periodic cells per side), for rows narrower than a block;
  *   [2] fused z kernel: 8, 16, 24, 32, 48 or 64 = rows per block of the
  *       marching kernel (one wave per RK stage); 8064 / 4128 = LDS tiles of
  *       8 columns x 64 rows / 4 columns x 128 rows (all stages in LDS);

@@ -18,7 +18,8 @@ from . import _lib
 from . import solve as S
 from ._lib import ptr, stream_of
 
-X_BLOCKS = (256, 448, 512)
+# x: threads per row tile, or 1000 + threads for dense tiles (rows end to end)
+X_BLOCKS = (256, 448, 512, 1256, 1512)
 # z: rows per marching segment, or an LDS tile code columns*1000 + rows
 # (8064, 4128: one cell per thread; 8128: two cells per thread)
 Z_SEGMENTS = (8, 16, 24, 32, 48, 64, 8064, 4128, 8128)

@@ -6,10 +6,11 @@
 // with a halo refresh of each forcing state. Each stage widens the stencil
 // by 2 cells, so an output tile needs S on the tile +- 6 cells: stage 1 is
 // computed on tile +- 4, stage 2 on tile +- 2, stage 3 on the tile. Per
-// interior cell and direction this reads S once and writes the result and
-// the stage-2 state (state_tmp): 96 B instead of the 256 B of three stage
-// launches. Every stage value is computed with the same operations as the
-// per-stage kernels, so results are bitwise identical to them.
+// interior cell and direction this reads S once and writes the result (and,
+// in the last direction of a call, the stage-2 state, state_tmp): 64 or 96 B
+// instead of the 256 B of three stage launches. Every stage value is computed
+// with the same operations as the per-stage kernels, so results are bitwise
+// identical to them.
 //
 // Buffers of a direction: `in` (its start state, read only), `out` (its
 // result interior, never aliases `in`), `st` (the reference's `state`: it
@@ -18,7 +19,8 @@
 //
 //  * x (k_dir_x<NT>): one block of NT threads per (row, tile of NT-12 cells).
 //    S, the running stage state and face fluxes live in LDS; periodic wrap
-//    replaces the halo columns at every stage.
+//    replaces the halo columns at every stage. k_dir_xd<NT>: the same over
+//    the rows laid end to end, so that narrow rows fill whole blocks.
 //  * z, LDS tiles (k_dir_zt<CW,NT>, the default): the x scheme turned on its
 //    side, CW columns x NT/CW rows per block, the wall halo rows of every
 //    stage state rebuilt from its edge row.
@@ -191,6 +193,109 @@ __global__ __launch_bounds__(NT) void k_dir_x(DirArgs a, Grid g, int ntiles, int
   if (logical >= nreal) return;  // whole block, before any barrier
   member_offset(a);
   dir_x_tile<NT>(a, g, logical / ntiles, (logical % ntiles) * C, sm);
+}
+
+// ---- x, dense tiles: the extended rows of a grid (cells -6 .. nx+5, E = nx+12
+// slots each, the periodic cells the three stages consume included) laid end
+// to end and cut into blocks of NT slots every NT-12. A row boundary inside a
+// block costs no more than the block boundary does, so narrow rows (200, 400
+// cells) fill the blocks instead of leaving a partial tile per row. A slot
+// holds a correct stage-s value when it lies 2s slots inside its block and
+// inside its own extended row (-6+2s <= cell <= nx+5-2s); values computed
+// across a row boundary are never written. Block windows [6, NT-6) partition
+// the slots, so every interior cell is written by one block; halo cells are
+// written wherever they are correct (the same value). Slot -> (row, cell)
+// takes one 32-bit division per thread.
+inline long long dense_x_slots(int nx, int nz) { return (long long)nz * (nx + 12); }
+inline int dense_x_blocks(int nx, int nz, int nt) {
+  return (int)((dense_x_slots(nx, nz) - 13) / (nt - 12)) + 1;
+}
+
+template <int NT>
+__device__ __forceinline__ void dir_xd_tile(const DirArgs& a, const Grid& g, int b,
+                                            XTileLds<NT>& sm) {
+  auto& s0 = sm.s0;
+  auto& s1 = sm.s1;
+  auto& fl = sm.fl;
+  const int nx = g.nx, nz = g.nz, t = threadIdx.x;
+  const unsigned E = (unsigned)nx + 12u;
+  const unsigned gs = (unsigned)b * (unsigned)(NT - 12) + (unsigned)t;
+  const int r = (int)(gs / E);
+  const int cell = (int)(gs - (unsigned)r * E) - 6;  // -6 .. nx+5
+  const bool row_ok = r < nz;
+  const int prow = (row_ok ? r : 0) + kHs;
+  const int cw = wrap_cell(cell, nx);
+  const bool own = row_ok && t >= 6 && t < NT - 6 && cell >= 0 && cell < nx;
+  const bool halo = row_ok && (cell == -2 || cell == -1 || cell == nx || cell == nx + 1);
+  {
+    double q[4] = {0.0, 0.0, 0.0, 0.0};
+    if (row_ok) {
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) q[v] = a.in[g.at(v, prow, cw + kHs)];
+    }
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) s0[v][t] = q[v];
+    if (halo) {
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) a.st[g.at(v, prow, cell + kHs)] = q[v];
+    }
+  }
+  // face j (between slots j-1 and j) is computed by thread j with its row's
+  // hydrostatic values; faces across a row boundary feed no correct cell
+  const double hyr = a.hy_cell[prow], hyt = a.hyt_cell[prow];
+  const double src = (a.src_w && row_ok) ? a.src_w[(long long)r * nx + cw] : 0.0;
+  const double id = 1.0 / a.d;
+  __syncthreads();
+#pragma unroll
+  for (int stage = 0; stage < 3; ++stage) {
+    const int m = 2 + 2 * stage;  // faces m..NT-m, cells m..NT-1-m
+    auto& src_s = stage == 0 ? s0 : s1;
+    if (t >= m && t <= NT - m) {
+      double v4[4], d3[4], F[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v)
+        recon4(src_s[v][t - 2], src_s[v][t - 1], src_s[v][t], src_s[v][t + 1], v4[v], d3[v]);
+      flux_x_face(v4, d3, hyr, hyt, a.hv, F);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) fl[v][t] = F[v];
+    }
+    __syncthreads();
+    if (t >= m && t <= NT - 1 - m) {
+      const double dts = stage == 0 ? a.dt1 : stage == 1 ? a.dt2 : a.dt3;
+      double q[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) q[v] = tend_of(fl[v][t], fl[v][t + 1], a.d, id);
+      if (a.src_w) q[2] = q[2] + src;
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) q[v] = s0[v][t] + dts * q[v];
+      if (stage < 2) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) s1[v][t] = q[v];  // in place: faces are done
+      }
+      if (stage == 0 && a.s1 && own) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) a.s1[g.at(v, prow, cell + kHs)] = q[v];
+      }
+      if (stage == 1 && ((own && a.tmp_interior) || halo)) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) a.tmp[g.at(v, prow, cell + kHs)] = q[v];
+      }
+      if (stage == 2 && own) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) a.out[g.at(v, prow, cell + kHs)] = q[v];
+      }
+    }
+    if (stage < 2) __syncthreads();
+  }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_dir_xd(DirArgs a, Grid g, int nreal, int per_xcd) {
+  __shared__ XTileLds<NT> sm;
+  const int logical = xcd_logical(blockIdx.x, per_xcd);
+  if (logical >= nreal) return;  // whole block, before any barrier
+  member_offset(a);
+  dir_xd_tile<NT>(a, g, logical, sm);
 }
 
 // ------------------------------------------------------------------ z ------
@@ -781,7 +886,7 @@ struct MemberDev {
   int nx, nz;
   double dx, dz, dt1, dt2, dt3, hvx, hvz;
   int x_start, z_start;  // first work item of this member in the x / z lists
-  int x_ntiles, z_ncol;  // x tiles per row, z column groups
+  int x_ntiles, z_ncol;  // x tiles per row (dense x tiles: blocks of the member), z column groups
 };
 
 __device__ __forceinline__ DirArgs member_args(const MemberDev& m, int dir, int pass,
@@ -840,6 +945,19 @@ __global__ __launch_bounds__(NT) void k_ens_x(const MemberDev* __restrict__ m, i
   const int row = local / md.x_ntiles;
   if (row >= md.nz) return;  // a plan that is not the prepared one: stay in bounds
   dir_x_tile<NT>(a, Grid(md.nx, md.nz), row, (local % md.x_ntiles) * C, sm);
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_ens_xd(const MemberDev* __restrict__ m, int n, int pass,
+                                               int total, int per_xcd) {
+  __shared__ XTileLds<NT> sm;
+  const int item = xcd_logical(blockIdx.x, per_xcd);
+  if (item >= total) return;  // whole block, before any barrier
+  const MemberDev& md = m[member_of<0>(m, n, item)];
+  const DirArgs a = member_args(md, 0, pass & 1, pass >> 1);
+  const int local = item - md.x_start;
+  if (local >= md.x_ntiles) return;  // as in k_ens_x
+  dir_xd_tile<NT>(a, Grid(md.nx, md.nz), local, sm);
 }
 
 template <int L>
@@ -907,6 +1025,15 @@ int launch_x(const DirArgs& a, Grid g, hipStream_t st, int members) {
   return PMW_OK;
 }
 
+template <int NT>
+int launch_xd(const DirArgs& a, Grid g, hipStream_t st, int members) {
+  const int nreal = dense_x_blocks(g.nx, g.nz, NT);
+  const int per = (nreal + kNumXcd - 1) / kNumXcd;
+  k_dir_xd<NT><<<dim3(per * kNumXcd, members), NT, 0, st>>>(a, g, nreal, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
 template <int CW, int NT>
 int launch_zt(const DirArgs& a, Grid g, hipStream_t st, int members) {
   constexpr int C = NT / CW - 12;
@@ -934,7 +1061,10 @@ int launch_z(const DirArgs& a, Grid g, hipStream_t st, int members) {
 
 namespace pmw {
 
-bool fused_x_tile_ok(int nt) { return nt == 256 || nt == 448 || nt == 512; }
+// x: 256, 448, 512 = threads per row tile; 1000 + 256 or 512 = dense tiles
+bool fused_x_tile_ok(int nt) {
+  return nt == 256 || nt == 448 || nt == 512 || nt == 1256 || nt == 1512;
+}
 // z: 8..64 = rows per marching segment; columns*1000 + rows = LDS tiles:
 // 8064, 4128 (one cell per thread), 8128 (two cells per thread)
 bool fused_z_seg_ok(int seg) {
@@ -944,10 +1074,13 @@ bool fused_z_seg_ok(int seg) {
 
 int launch_dir_x(const DirArgs& a, Grid g, int nt, hipStream_t st, int members) {
   if (members < 1 || members > 65535 || (members > 1 && a.mstride <= 0)) return PMW_EINVAL;
+  if (nt > 1000 && dense_x_slots(g.nx, g.nz) > (1LL << 30)) return PMW_EINVAL;
   switch (nt) {
     case 256: return launch_x<256>(a, g, st, members);
     case 448: return launch_x<448>(a, g, st, members);
     case 512: return launch_x<512>(a, g, st, members);
+    case 1256: return launch_xd<256>(a, g, st, members);
+    case 1512: return launch_xd<512>(a, g, st, members);
     default: return PMW_EINVAL;
   }
 }
@@ -990,7 +1123,10 @@ int plan_members(const pmw_member* mem, int n, int x_nt, int z_seg, MemberDev* d
       return PMW_EINVAL;
     if (m.state == m.state_tmp || m.state == m.state_work || m.state_tmp == m.state_work)
       return PMW_EALIAS;
-    const int xt = (m.nx + ens_x_cells(x_nt) - 1) / ens_x_cells(x_nt);
+    const bool dense = x_nt > 1000;
+    if (dense && dense_x_slots(m.nx, m.nz) > (1LL << 30)) return PMW_EINVAL;
+    const int xt = dense ? dense_x_blocks(m.nx, m.nz, x_nt - 1000)
+                         : (m.nx + ens_x_cells(x_nt) - 1) / ens_x_cells(x_nt);
     int zc, zsg;
     if (z_seg >= 1000) {  // LDS tiles: columns*1000 + rows (12 of them halo)
       const int cw = z_seg / 1000, rows = z_seg % 1000 - 12;
@@ -1025,7 +1161,7 @@ int plan_members(const pmw_member* mem, int n, int x_nt, int z_seg, MemberDev* d
       d.x_ntiles = xt;
       d.z_ncol = zc;
     }
-    xs += (long long)xt * m.nz;
+    xs += dense ? (long long)xt : (long long)xt * m.nz;  // dense: blocks, else tiles x rows
     zs += (long long)zc * zsg;
     if (xs > (1LL << 30) || zs > (1LL << 30)) return PMW_EINVAL;
   }
@@ -1048,6 +1184,14 @@ template <int NT>
 int launch_ens_x(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
   const int per = ens_per_xcd(total);
   k_ens_x<NT><<<per * kNumXcd, NT, 0, st>>>(d, n, pass, total, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
+template <int NT>
+int launch_ens_xd(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
+  const int per = ens_per_xcd(total);
+  k_ens_xd<NT><<<per * kNumXcd, NT, 0, st>>>(d, n, pass, total, per);
   PMW_CHECK_LAUNCH();
   return PMW_OK;
 }
@@ -1081,6 +1225,8 @@ int launch_ens(int dir, int x_nt, int z_seg, const MemberDev* d, int n, int pass
   if (dir == 0) {
     if (x_nt == 256) return launch_ens_x<256>(d, n, pass, plan.x_total, st);
     if (x_nt == 448) return launch_ens_x<448>(d, n, pass, plan.x_total, st);
+    if (x_nt == 1256) return launch_ens_xd<256>(d, n, pass, plan.x_total, st);
+    if (x_nt == 1512) return launch_ens_xd<512>(d, n, pass, plan.x_total, st);
     return launch_ens_x<512>(d, n, pass, plan.x_total, st);
   }
   switch (z_seg) {

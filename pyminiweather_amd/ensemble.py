@@ -57,7 +57,7 @@ def mixed_thermal_members(n=256, seed=0, grids=GRID_CHOICES):
 
 
 class Ensemble:
-    def __init__(self, params_list, bubbles=None, device=None, tuning=(0, 512, 4128)):
+    def __init__(self, params_list, bubbles=None, device=None, tuning=(0, 1256, 8128)):
         if device is None:
             device = torch.device("cuda", torch.cuda.current_device())
         self.device = device
