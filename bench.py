@@ -68,7 +68,9 @@ def fused_kernel_names(tuning):
     """The x and z kernels (rocprof names) that a fused tuning launches."""
     _, xb, zs = tuning
     x = f"k_dir_xd<{xb - 1000}>" if xb > 1000 else f"k_dir_x<{xb}>"
-    if zs >= 1000:
+    if zs == 18128:
+        z = "k_dir_zt2d<8, 128>"
+    elif zs >= 1000:
         cols, rows = divmod(zs, 1000)
         z = f"k_dir_zt2<{cols}, {rows}>" if zs == 8128 else f"k_dir_zt<{cols}, {cols * rows}>"
     else:

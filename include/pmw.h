@@ -190,7 +190,9 @@ int pmw_evolve(double* state, double* state_tmp, double* state_work,
  *   [2] fused z kernel: 8, 16, 24, 32, 48 or 64 = rows per block of the
  *       marching kernel (one wave per RK stage); 8064 / 4128 = LDS tiles of
  *       8 columns x 64 rows / 4 columns x 128 rows (all stages in LDS);
- *       8128 = LDS tiles of 8 columns x 128 rows, two cells per thread.
+ *       8128 = LDS tiles of 8 columns x 128 rows, two cells per thread;
+ *       18128 = the 8128 tiles over the column groups laid end to end
+ *       (each column with its two wall halo rows per side).
  * Both modes leave bit-identical `state` and `state_tmp`.
  * `kernel_ms` (HOST, 2 doubles, or NULL): if given, a HIP event is recorded
  * around every launch, the call blocks until they finish, and each launch's

@@ -21,9 +21,10 @@
 //    S, the running stage state and face fluxes live in LDS; periodic wrap
 //    replaces the halo columns at every stage. k_dir_xd<NT>: the same over
 //    the rows laid end to end, so that narrow rows fill whole blocks.
-//  * z, LDS tiles (k_dir_zt<CW,NT>, the default): the x scheme turned on its
-//    side, CW columns x NT/CW rows per block, the wall halo rows of every
-//    stage state rebuilt from its edge row.
+//  * z, LDS tiles (k_dir_zt<CW,NT>; k_dir_zt2<CW,NR> with two cells per
+//    thread, the default): the x scheme turned on its side, CW columns per
+//    block, the wall halo rows of every stage state rebuilt from its edge row.
+//    k_dir_zt2d: the same over the column groups laid end to end.
 //  * z, marching (k_dir_z<L>): a block is three waves over 64 columns and a
 //    segment of L rows; wave s runs RK stage s+1 one row step behind the
 //    previous stage, rows handed over through a 2-slot LDS ring with one
@@ -860,6 +861,171 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
   }
 }
 
+// ---- z, dense tiles: the extended columns of the CW-column groups (rows
+// -2 .. nz+1: the wall halo rows are rebuilt from the edge row at every
+// stage, so a column needs no more) laid end to end, slot row g of block b
+// being b*(NR-12) + j - 4, cut into tiles of NR rows; the offset puts row 0
+// of the first group at the first window row. A slot row is (group, row);
+// the two cells of a thread may lie in different groups. Windows [6, NR-6)
+// partition the interior rows; the wall halo rows written to st and tmp
+// belong to the block whose window holds their edge row. A halo row whose
+// edge row lies outside the block is left unrebuilt: no correct cell of the
+// block reaches it.
+inline long long dense_z_rows(int nx, int nz, int cw) {
+  return (long long)((nx + cw - 1) / cw) * (nz + 4);
+}
+inline int dense_z_blocks(int nx, int nz, int cw, int nr) {
+  return (int)((dense_z_rows(nx, nz, cw) - 5) / (nr - 12)) + 1;
+}
+
+template <int CW, int NR>
+__device__ __forceinline__ void dir_zt2d_tile(const DirArgs& a, const Grid& g, int b,
+                                              ZTile2Lds<CW * NR>& sm, bool corners) {
+  constexpr int NC = CW * NR, NT = NC / 2, C = NR - 12;
+  auto& s = sm.s;
+  auto& fl = sm.fl;
+  const int nx = g.nx, nz = g.nz, t = threadIdx.x;
+  const int Ez = nz + 4;
+  const int col = t % CW;
+  const gdouble* hy = a.hy_cell;
+  const int plo = kHs, phi = nz + kHs - 1;
+  int jj[2], kk[2], sl[2], es[2], pc[2], ii[2];
+  bool colok[2], wl[2], wh[2], own[2], ownh[2], esok[2];
+  double S[2][4], src[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    sl[c] = t + c * NT;
+    jj[c] = sl[c] / CW;
+    const int gr = b * C + jj[c] - 4;
+    const int grp = gr >= 0 ? gr / Ez : -1;
+    kk[c] = gr >= 0 ? gr - grp * Ez - 2 : -3;  // -3: padding before the first group
+    ii[c] = grp * CW + col;
+    colok[c] = gr >= 0 && ii[c] < nx;
+    pc[c] = (colok[c] ? ii[c] : 0) + kHs;
+    wl[c] = kk[c] == -2 || kk[c] == -1;
+    wh[c] = kk[c] == nz || kk[c] == nz + 1;
+    own[c] = colok[c] && jj[c] >= 6 && jj[c] < NR - 6 && kk[c] >= 0 && kk[c] < nz;
+    const int je = wl[c] ? jj[c] - kk[c] : jj[c] - (kk[c] - (nz - 1));  // edge row's tile row
+    esok[c] = (wl[c] || wh[c]) && je >= 0 && je < NR;
+    es[c] = esok[c] ? je * CW + col : 0;
+    ownh[c] = colok[c] && (wl[c] || wh[c]) && je >= 6 && je < NR - 6;
+    double q[4] = {0.0, 0.0, 0.0, 0.0};
+    if (colok[c]) {
+      if (kk[c] >= 0 && kk[c] < nz) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) q[v] = a.in[g.at(v, kk[c] + kHs, pc[c])];
+      } else if (wl[c] || wh[c]) {
+        const int pe = wl[c] ? plo : phi;
+#pragma unroll
+        for (int v = 0; v < kNv; ++v)
+          q[v] = wall_halo(v, a.in[g.at(v, pe, pc[c])], hy[pe], hy[kk[c] + kHs]);
+        if (ownh[c]) {  // set_bc_z of stage 1 into st
+#pragma unroll
+          for (int v = 0; v < kNv; ++v) a.st[g.at(v, kk[c] + kHs, pc[c])] = q[v];
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) {
+      s[v][sl[c]] = q[v];
+      S[c][v] = q[v];
+    }
+    src[c] = (a.src_w && colok[c] && kk[c] >= 0 && kk[c] < nz)
+                 ? a.src_w[(long long)kk[c] * nx + ii[c]] : 0.0;
+  }
+  if (corners && t < 8) {
+    // corner columns (padded 0, 1, nx+2, nx+3) of the halo rows of st, tmp
+    gdouble* buf = t < 4 ? a.st : a.tmp;
+    const int qq = t & 3;
+    const int cc = qq < 2 ? qq : nx + qq;
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) {
+      const double e = buf[g.at(v, plo, cc)];
+      buf[g.at(v, 0, cc)] = wall_halo(v, e, hy[plo], hy[0]);
+      buf[g.at(v, 1, cc)] = wall_halo(v, e, hy[plo], hy[1]);
+      const double f = buf[g.at(v, phi, cc)];
+      buf[g.at(v, nz + 2, cc)] = wall_halo(v, f, hy[phi], hy[nz + 2]);
+      buf[g.at(v, nz + 3, cc)] = wall_halo(v, f, hy[phi], hy[nz + 3]);
+    }
+  }
+  gdouble* const tmp_own = a.tmp_interior ? a.tmp : nullptr;
+  const double id = 1.0 / a.d;
+  __syncthreads();
+#pragma unroll
+  for (int stage = 0; stage < 3; ++stage) {
+    const int e = 2 + 2 * stage;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int j = jj[c], k = kk[c], u = sl[c];
+      if (colok[c] && j >= e && j <= NR - e && k >= 0 && k <= nz) {
+        double v4[4], d3[4], G[4];
+#pragma unroll
+        for (int v = 0; v < kNv; ++v)
+          recon4(s[v][u - 2 * CW], s[v][u - CW], s[v][u], s[v][u + CW], v4[v], d3[v]);
+        flux_z_face(v4, d3, a.hyr_int[k], a.hyt_int[k], a.hyp_int[k], a.hv, k == 0 || k == nz,
+                    G);
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) fl[v][u] = G[v];
+      }
+    }
+    __syncthreads();
+    const double dts = stage == 0 ? a.dt1 : stage == 1 ? a.dt2 : a.dt3;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int j = jj[c], k = kk[c], u = sl[c];
+      if (colok[c] && j >= e && j <= NR - e - 1 && k >= 0 && k < nz) {
+        double td[4];
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) td[v] = tend_of(fl[v][u], fl[v][u + CW], a.d, id);
+        td[2] = td[2] - s[0][u] * kGrav;
+        if (a.src_w) td[2] = td[2] + src[c];
+        double q[4];
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) q[v] = S[c][v] + dts * td[v];
+        if (stage < 2) {
+#pragma unroll
+          for (int v = 0; v < kNv; ++v) s[v][u] = q[v];  // in place: faces are done
+        }
+        if (own[c]) {
+          gdouble* dst = stage == 0 ? a.s1 : stage == 1 ? tmp_own : a.out;
+          if (dst) {
+#pragma unroll
+            for (int v = 0; v < kNv; ++v) dst[g.at(v, k + kHs, pc[c])] = q[v];
+          }
+        }
+      }
+    }
+    if (stage == 2) break;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (colok[c] && esok[c]) {
+        const int pe = wl[c] ? plo : phi;
+        double q[4];
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) q[v] = wall_halo(v, s[v][es[c]], hy[pe], hy[kk[c] + kHs]);
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) s[v][sl[c]] = q[v];
+        if (stage == 1 && ownh[c]) {  // set_bc_z of stage 3
+#pragma unroll
+          for (int v = 0; v < kNv; ++v) a.tmp[g.at(v, kk[c] + kHs, pc[c])] = q[v];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int CW, int NR>
+__global__ __launch_bounds__(CW * NR / 2) void k_dir_zt2d(DirArgs a, Grid g, int nreal,
+                                                          int per_xcd) {
+  __shared__ ZTile2Lds<CW * NR> sm;
+  const int logical = xcd_logical(blockIdx.x, per_xcd);
+  if (logical >= nreal) return;  // whole block, before any barrier
+  member_offset(a);
+  dir_zt2d_tile<CW, NR>(a, g, logical, sm, logical == 0);
+}
+
 template <int CW, int NR>
 __global__ __launch_bounds__(CW * NR / 2) void k_dir_zt2(DirArgs a, Grid g, int ncolgrp,
                                                          int nreal, int per_xcd) {
@@ -989,6 +1155,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 }
 
 template <int CW, int NR>
+__global__ __launch_bounds__(CW * NR / 2) void k_ens_zt2d(const MemberDev* __restrict__ m, int n,
+                                                          int pass, int total, int per_xcd) {
+  __shared__ ZTile2Lds<CW * NR> sm;
+  const int item = xcd_logical(blockIdx.x, per_xcd);
+  if (item >= total) return;  // whole block, before any barrier
+  const MemberDev& md = m[member_of<1>(m, n, item)];
+  const DirArgs a = member_args(md, 1, pass & 1, pass >> 1);
+  const int local = item - md.z_start;
+  if (local >= md.z_ncol) return;  // dense z: z_ncol holds the member's blocks
+  dir_zt2d_tile<CW, NR>(a, Grid(md.nx, md.nz), local, sm, local == 0);
+}
+
+template <int CW, int NR>
 __global__ __launch_bounds__(CW * NR / 2) void k_ens_zt2(const MemberDev* __restrict__ m, int n,
                                                          int pass, int total, int per_xcd) {
   __shared__ ZTile2Lds<CW * NR> sm;
@@ -1010,6 +1189,15 @@ int launch_zt2(const DirArgs& a, Grid g, hipStream_t st, int members) {
   const int per = (int)((nreal + kNumXcd - 1) / kNumXcd);
   k_dir_zt2<CW, NR><<<dim3(per * kNumXcd, members), CW * NR / 2, 0, st>>>(a, g, ncolgrp,
                                                                           (int)nreal, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
+template <int CW, int NR>
+int launch_zt2d(const DirArgs& a, Grid g, hipStream_t st, int members) {
+  const int nreal = dense_z_blocks(g.nx, g.nz, CW, NR);
+  const int per = (nreal + kNumXcd - 1) / kNumXcd;
+  k_dir_zt2d<CW, NR><<<dim3(per * kNumXcd, members), CW * NR / 2, 0, st>>>(a, g, nreal, per);
   PMW_CHECK_LAUNCH();
   return PMW_OK;
 }
@@ -1066,10 +1254,11 @@ bool fused_x_tile_ok(int nt) {
   return nt == 256 || nt == 448 || nt == 512 || nt == 1256 || nt == 1512;
 }
 // z: 8..64 = rows per marching segment; columns*1000 + rows = LDS tiles:
-// 8064, 4128 (one cell per thread), 8128 (two cells per thread)
+// 8064, 4128 (one cell per thread), 8128 (two cells per thread); 18128 =
+// 8128 as dense tiles (column groups end to end)
 bool fused_z_seg_ok(int seg) {
   return seg == 8 || seg == 16 || seg == 24 || seg == 32 || seg == 48 || seg == 64 ||
-         seg == 8064 || seg == 4128 || seg == 8128;
+         seg == 8064 || seg == 4128 || seg == 8128 || seg == 18128;
 }
 
 int launch_dir_x(const DirArgs& a, Grid g, int nt, hipStream_t st, int members) {
@@ -1097,6 +1286,9 @@ int launch_dir_z(const DirArgs& a, Grid g, int seg, hipStream_t st, int members)
     case 8064: return launch_zt<8, 512>(a, g, st, members);
     case 4128: return launch_zt<4, 512>(a, g, st, members);
     case 8128: return launch_zt2<8, 128>(a, g, st, members);
+    case 18128:
+      if (dense_z_rows(g.nx, g.nz, 8) > (1LL << 30)) return PMW_EINVAL;
+      return launch_zt2d<8, 128>(a, g, st, members);
     default: return PMW_EINVAL;
   }
 }
@@ -1128,7 +1320,11 @@ int plan_members(const pmw_member* mem, int n, int x_nt, int z_seg, MemberDev* d
     const int xt = dense ? dense_x_blocks(m.nx, m.nz, x_nt - 1000)
                          : (m.nx + ens_x_cells(x_nt) - 1) / ens_x_cells(x_nt);
     int zc, zsg;
-    if (z_seg >= 1000) {  // LDS tiles: columns*1000 + rows (12 of them halo)
+    if (z_seg == 18128) {  // dense tiles: zc = the member's blocks, one "segment"
+      if (dense_z_rows(m.nx, m.nz, 8) > (1LL << 30)) return PMW_EINVAL;
+      zc = dense_z_blocks(m.nx, m.nz, 8, 128);
+      zsg = 1;
+    } else if (z_seg >= 1000) {  // LDS tiles: columns*1000 + rows (12 of them halo)
       const int cw = z_seg / 1000, rows = z_seg % 1000 - 12;
       zc = (m.nx + cw - 1) / cw;
       zsg = (m.nz + rows - 1) / rows;
@@ -1205,6 +1401,14 @@ int launch_ens_z(const MemberDev* d, int n, int pass, int total, hipStream_t st)
 }
 
 template <int CW, int NR>
+int launch_ens_zt2d(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
+  const int per = ens_per_xcd(total);
+  k_ens_zt2d<CW, NR><<<per * kNumXcd, CW * NR / 2, 0, st>>>(d, n, pass, total, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
+template <int CW, int NR>
 int launch_ens_zt2(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
   const int per = ens_per_xcd(total);
   k_ens_zt2<CW, NR><<<per * kNumXcd, CW * NR / 2, 0, st>>>(d, n, pass, total, per);
@@ -1238,6 +1442,7 @@ int launch_ens(int dir, int x_nt, int z_seg, const MemberDev* d, int n, int pass
     case 8064: return launch_ens_zt<8, 512>(d, n, pass, plan.z_total, st);
     case 4128: return launch_ens_zt<4, 512>(d, n, pass, plan.z_total, st);
     case 8128: return launch_ens_zt2<8, 128>(d, n, pass, plan.z_total, st);
+    case 18128: return launch_ens_zt2d<8, 128>(d, n, pass, plan.z_total, st);
     default: return launch_ens_z<64>(d, n, pass, plan.z_total, st);
   }
 }
