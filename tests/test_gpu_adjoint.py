@@ -162,16 +162,18 @@ def test_gradient_matches_torch_autograd(ic, nx, nz, steps):
     assert r <= 1e-11, f"gradient rel L2 {r:.3e}"
 
 
-def test_tape_forward_bitwise_equals_evolve():
+@pytest.mark.parametrize("tape_tuning", [None, (0, 1256, 18128), (0, 1512, 8064)])
+def test_tape_forward_bitwise_equals_evolve(tape_tuning):
     """Every state slot the tape keeps equals the state the fused evolve
-    passes through (two members, both direction orders)."""
+    passes through (two members, both direction orders), whatever kernel
+    shapes the tape runs (row or dense tiles)."""
     import ctypes
     from pyminiweather_amd import _lib
     from pyminiweather_amd.adjoint import AdjointSolver
     import pyminiweather_amd.solve as S
     p = O.make_params(70, 33, "collision", cfl=1.5, max_speed=450.0)
     f = _fields(p)
-    solver = AdjointSolver(p, f, 3, checkpoint_every=3, members=2)
+    solver = AdjointSolver(p, f, 3, checkpoint_every=3, members=2, tuning=tape_tuning)
     s0 = torch.stack([f.state, f.state.clone()])
     s0[1, :, 2:-2, 2:-2] += _perturbed_interior(f, 4) - f.state[:, 2:-2, 2:-2]
     solver._tape_run(s0, 1, 3, store=True)
@@ -180,7 +182,7 @@ def test_tape_forward_bitwise_equals_evolve():
         st, tmp, work = s0[m].clone(), s0[m].clone(), torch.zeros_like(s0[m])
         for k in range(3):
             flag = ctypes.c_int((1 + k) % 2)
-            tune = (ctypes.c_int * 3)(*solver.tuning)
+            tune = (ctypes.c_int * 3)(*S.DEFAULT_TUNING)
             _lib.call("pmw_evolve_ex", _lib.ptr(st), _lib.ptr(tmp), _lib.ptr(work),
                       _lib.ptr(f.hy_dens_cell), _lib.ptr(f.hy_dens_theta_cell),
                       _lib.ptr(f.hy_dens_int), _lib.ptr(f.hy_dens_theta_int),
