@@ -149,17 +149,17 @@ def test_ensemble_rejects_bad_tuning_and_injection():
         Ensemble([member_params(20, 10, "injection")])
 
 
-def test_full_256_member_ensemble():
-    """The seeded 256-member mixed-size ensemble at full size, 6 steps: every
-    member conserves mass to 1e-13 relative with finite energy, and the
-    first member of each grid size equals its own single-grid solve bit for
-    bit."""
+@pytest.mark.parametrize("tuning", [(0, 256, 8128), (0, 1256, 18128)])
+def test_full_256_member_ensemble(tuning):
+    """The seeded 256-member mixed-size ensemble at full size, 6 steps, with
+    row tiles and with dense tiles: every member conserves mass to 1e-13
+    relative with finite energy, and the first member of each grid size
+    equals its own single-grid solve (row tiles) bit for bit."""
     import pyminiweather_amd.solve as S
     from pyminiweather_amd.data import initialize_fields
     from pyminiweather_amd.ensemble import GRID_CHOICES, Ensemble, mixed_thermal_members
     from pyminiweather_amd.ics import init
     from pyminiweather_amd.mesh import MeshData
-    tuning = (0, 256, 8128)
     params, bubbles = mixed_thermal_members(256, seed=0)
     ens = Ensemble(params, bubbles, tuning=tuning)
     assert len(ens) == 256 and {(p["nx"], p["nz"]) for p in params} == set(GRID_CHOICES)
@@ -178,7 +178,7 @@ def test_full_256_member_ensemble():
         mesh = MeshData(p)
         init(f, p, mesh, bubbles=[bubbles[q]])
         S.reverse_direction = False
-        S.set_tuning(nx, nz, tuning)
+        S.set_tuning(nx, nz, (0, 512, 8128))
         try:
             S.run(p, f, mesh, 6)
         finally:
