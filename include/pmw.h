@@ -236,16 +236,20 @@ long long pmw_ensemble_workspace_size(int nmembers);
 
 /* Uploads the member descriptors and work-list offsets for `tuning` (same
  * format as pmw_evolve_ex; mode must be 0) into `workspace` and waits for the
- * copy. Call again whenever members or tuning change. */
+ * copy. Call again whenever members or tuning change. The library records
+ * (on the host, keyed by the workspace address) which members and tuning a
+ * workspace holds. Rewriting a workspace that a captured HIP graph uses
+ * changes what the graph's launches read: give each tuning its own
+ * workspace instead. */
 int pmw_ensemble_prepare(const pmw_member* members, int nmembers, const int* tuning,
                          void* workspace, long long workspace_bytes, void* stream);
 
 /* evolve (pyminiweather/solve/step.py:85-143) for every member at once:
  * `nsteps` timesteps, each one launch per direction over the concatenated
  * tile lists of all members (the three RK stages fused as in
- * pmw_evolve_ex). `members`
- * and `tuning` must be what was prepared (with others the states are wrong,
- * but every tile stays inside its member's rows). Only kernel launches are issued,
+ * pmw_evolve_ex). `members` and `tuning` must be what `workspace` was last
+ * prepared with: otherwise (or if it was never prepared) PMW_EINVAL and
+ * nothing is launched. Only kernel launches are issued,
  * so a call can be captured in a HIP graph. Each member ends bit-identical
  * to pmw_evolve on that member alone. kernel_ms as in pmw_evolve_ex (each
  * launch is synchronised when given). */

@@ -32,7 +32,9 @@
 //    lanes past nx write the corner columns of the halo rows.
 //  * k_ens_*: the same tiles over the flattened tile list of an ensemble.
 // Block ids are remapped so that each XCD runs a contiguous range of tiles.
+#include <mutex>
 #include <new>
+#include <unordered_map>
 
 #include "pmw_common.h"
 #include "pmw_fused.h"
@@ -1379,6 +1381,42 @@ int ens_tuning(const int* t, int& x_nt, int& z_seg) {
   return PMW_OK;
 }
 
+// What a workspace was prepared for: pmw_ensemble_evolve refuses members or
+// a tuning other than those (the device descriptors and work-list offsets
+// would not match the launches). Kept on the host, keyed by the workspace's
+// address, so that checking it needs no device read (a call stays
+// capturable in a HIP graph).
+struct EnsPrepared {
+  int nmembers, x_nt, z_seg, x_total, z_total;
+  unsigned long long members_hash;
+  bool operator==(const EnsPrepared& o) const {
+    return nmembers == o.nmembers && x_nt == o.x_nt && z_seg == o.z_seg &&
+           x_total == o.x_total && z_total == o.z_total &&
+           members_hash == o.members_hash;
+  }
+};
+
+std::mutex g_prepared_mu;
+std::unordered_map<const void*, EnsPrepared> g_prepared;
+
+// FNV-1a over the member descriptors (plain pointers, ints and doubles with
+// no padding between them).
+unsigned long long hash_members(const pmw_member* m, int n) {
+  const unsigned char* b = reinterpret_cast<const unsigned char*>(m);
+  const size_t len = sizeof(pmw_member) * (size_t)n;
+  unsigned long long h = 1469598103934665603ULL;
+  for (size_t i = 0; i < len; ++i) {
+    h ^= b[i];
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
+EnsPrepared prepared_for(const pmw_member* m, int n, int x_nt, int z_seg,
+                         const EnsPlan& plan) {
+  return EnsPrepared{n, x_nt, z_seg, plan.x_total, plan.z_total, hash_members(m, n)};
+}
+
 inline int ens_per_xcd(int total) { return (total + kNumXcd - 1) / kNumXcd; }
 
 template <int NT>
@@ -1482,9 +1520,18 @@ int pmw_ensemble_prepare(const pmw_member* members, int nmembers, const int* tun
       starts[nmembers + q] = dev[q].z_start;
     }
     hipStream_t st = (hipStream_t)stream;
+    {
+      // until the copy has landed the workspace matches no tuning
+      std::lock_guard<std::mutex> lock(g_prepared_mu);
+      g_prepared.erase(workspace);
+    }
     hipError_t e = hipMemcpyAsync(workspace, host, bytes, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = (int)e;
+    if (rc == PMW_OK) {
+      std::lock_guard<std::mutex> lock(g_prepared_mu);
+      g_prepared[workspace] = prepared_for(members, nmembers, x_nt, z_seg, plan);
+    }
   }
   delete[] host;
   return rc;
@@ -1502,6 +1549,13 @@ int pmw_ensemble_evolve(const pmw_member* members, int nmembers, const int* tuni
   EnsPlan plan;
   rc = plan_members(members, nmembers, x_nt, z_seg, nullptr, plan);
   if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lock(g_prepared_mu);
+    const auto it = g_prepared.find(workspace);
+    if (it == g_prepared.end() ||
+        !(it->second == prepared_for(members, nmembers, x_nt, z_seg, plan)))
+      return PMW_EINVAL;  // not prepared, or prepared for other members / tuning
+  }
   const MemberDev* d = (const MemberDev*)workspace;
   hipStream_t st = (hipStream_t)stream;
   hipEvent_t ev[2] = {nullptr, nullptr};

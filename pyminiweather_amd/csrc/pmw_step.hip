@@ -62,6 +62,22 @@ __global__ void k_halo_x(double* __restrict__ s, Grid g, int bc,
   for (int v = 0; v < kNv; ++v) s[g.at(v, prow, c)] = q[v];
 }
 
+// Right x-halo columns nx+2, nx+3 of the interior rows, all 4 variables,
+// from `src` to `dst`. With the injection inflow set_bc_x leaves that halo as
+// the forcing buffer holds it (bcs.py:37), and the reference's stage 2 reads
+// state_tmp's (step.py:122-131); this build's stage 2 reads state_work, so
+// state_tmp's right halo is copied there first.
+__global__ void k_copy_right_halo(const double* __restrict__ src,
+                                  double* __restrict__ dst, Grid g) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // (var, row, slot)
+  if (idx >= kNv * g.nz * 2) return;
+  const int slot = idx & 1;
+  const int k = (idx >> 1) % g.nz;
+  const int v = (idx >> 1) / g.nz;
+  const long long a = g.at(v, k + kHs, g.nx + kHs + slot);
+  dst[a] = src[a];
+}
+
 __global__ void k_halo_z(double* __restrict__ s, Grid g,
                          const double* __restrict__ hy) {
   // one thread per column 0..nx+3; writes rows 0,1,nz+2,nz+3
@@ -699,6 +715,11 @@ int evolve_impl(double* state, double* state_tmp, double* state_work,
       double* forc[3] = {state, state_work, state_tmp};
       double* outs[3] = {state_work, state_tmp, state};
       for (int s = 0; s < 3; ++s) {
+        if (s == 1 && dir == 0 && bc == PMW_BC_INJECTION) {
+          const int n = kNv * nz * 2;
+          k_copy_right_halo<<<(n + 255) / 256, 256, 0, st>>>(state_tmp, state_work, g);
+          PMW_CHECK_LAUNCH();
+        }
         int rc = rec(0, dir);
         if (rc) return rc;
         if (dir == 0)

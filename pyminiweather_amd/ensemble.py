@@ -87,8 +87,6 @@ class Ensemble:
             m.nx, m.nz = f.nx, f.nz
             m.dx, m.dz = float(p["dx"]), float(p["dz"])
             m.dt = m.dt_full = float(p["dt"])
-        nbytes = int(_lib.load().pmw_ensemble_workspace_size(n))
-        self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=device)
         self.reverse_direction = False
         self.set_tuning(tuning)
 
@@ -100,11 +98,16 @@ class Ensemble:
         return sum(f.nx * f.nz for f in self.fields)
 
     def set_tuning(self, tuning):
+        """Prepare a fresh workspace for `tuning`. Graphs captured earlier
+        keep (and hold a reference to) the workspace they were captured
+        with, whose descriptors match their launches."""
         self.tuning = tuple(int(v) for v in tuning)
         self._tune = (ctypes.c_int * 3)(*self.tuning)
+        nbytes = int(_lib.load().pmw_ensemble_workspace_size(len(self)))
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
         _lib.call("pmw_ensemble_prepare", self.members, len(self), self._tune,
-                  ctypes.c_void_p(self.workspace.data_ptr()), self.workspace.numel(),
-                  stream_of(self.device))
+                  ctypes.c_void_p(ws.data_ptr()), ws.numel(), stream_of(self.device))
+        self.workspace = ws
 
     def evolve(self, nsteps=1, kernel_ms=None):
         """`nsteps` timesteps of every member (each with its own dt)."""
@@ -126,6 +129,7 @@ class Ensemble:
             self.evolve(2 * step_pairs)
         torch.cuda.current_stream(self.device).wait_stream(s)
         assert self.reverse_direction == saved
+        g.pmw_workspace = self.workspace  # the descriptors its launches read
         return g
 
     def stats(self):

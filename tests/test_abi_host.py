@@ -95,6 +95,18 @@ def test_argument_errors_return_codes_without_launch(lib):
     assert need > 0
     assert lib.pmw_compute_stats(FAKE, FAKE, FAKE, 2000, 1000, 10.0, 10.0, P2, need - 1,
                                  P3, None) == -3
+    # an ensemble workspace that was never prepared is refused before any launch
+    from pyminiweather_amd._lib import PmwMember
+    mem = (PmwMember * 1)()
+    m = mem[0]
+    m.state, m.state_tmp, m.state_work = 0x10000, 0x20000, 0x30000
+    m.hy_dens_cell = m.hy_dens_theta_cell = m.hy_dens_int = 0x40000
+    m.hy_dens_theta_int = m.hy_pressure_int = 0x40000
+    m.nx, m.nz, m.dx, m.dz, m.dt, m.dt_full = 20, 10, 1000.0, 1000.0, 1.0, 1.0
+    tune = (ctypes.c_int * 3)(0, 256, 16)
+    assert lib.pmw_ensemble_evolve(mem, 1, tune, ctypes.c_void_p(0x50000),
+                                   ctypes.byref(flag), 1, None, None) == -1
+    assert flag.value == 0
     # bad IC kind / too many bubbles
     geo = (ctypes.c_double * 16)()
     assert lib.pmw_init(9, FAKE, FAKE, FAKE, FAKE, FAKE, FAKE, None, 10, 10, geo, None, 0,

@@ -140,6 +140,41 @@ def test_graph_replays_match_eager(pairs):
                                           _host(b.fields[q].state_tmp))
 
 
+def test_graph_keeps_its_workspace_across_set_tuning():
+    """A graph captured before set_tuning() replays its own tile plan over
+    the workspace it was captured with (each tuning gets a fresh one), so it
+    still matches an eager twin bit for bit; and a workspace is refused for a
+    tuning it was not prepared with."""
+    import ctypes
+    from pyminiweather_amd import _lib
+    a = _ensemble((0, 256, 16))
+    b = _ensemble((0, 256, 16))
+    g = a.capture(step_pairs=1)
+    old_ws = a.workspace
+    a.set_tuning((0, 1256, 8128))
+    assert a.workspace.data_ptr() != old_ws.data_ptr()
+    g.replay()
+    b.evolve(2)
+    torch.cuda.synchronize()
+    for q in range(len(MEMBERS)):
+        np.testing.assert_array_equal(_host(a.fields[q].state), _host(b.fields[q].state))
+    # the new workspace with the old tuning: refused, nothing launched
+    flag = ctypes.c_int(0)
+    old_tune = (ctypes.c_int * 3)(0, 256, 16)
+    before = _host(a.fields[0].state)
+    with pytest.raises(_lib.PMWError) as err:
+        _lib.call("pmw_ensemble_evolve", a.members, len(a), old_tune,
+                  ctypes.c_void_p(a.workspace.data_ptr()), ctypes.byref(flag), 1,
+                  _lib.stream_of(), None)
+    assert err.value.code == -1 and flag.value == 0
+    np.testing.assert_array_equal(_host(a.fields[0].state), before)
+    # and the new tuning still advances both like single steps
+    a.evolve(2)
+    b.evolve(2)
+    for q in range(len(MEMBERS)):
+        np.testing.assert_array_equal(_host(a.fields[q].state), _host(b.fields[q].state))
+
+
 def test_ensemble_rejects_bad_tuning_and_injection():
     from pyminiweather_amd import _lib
     from pyminiweather_amd.ensemble import Ensemble, member_params

@@ -332,6 +332,30 @@ def test_evolve_matches_reference(ic, nx, nz, steps):
     assert S.reverse_direction == bool(g["reverse_direction_end"])
 
 
+def test_injection_from_random_state_with_halos():
+    """With the injection inflow, set_bc_x leaves the right x halo as each
+    forcing buffer holds it (ics/bcs.py:37-64), so every stage reads the
+    halo of the buffer the reference passes (step.py:111-141; stage 2 reads
+    state_tmp's). Starting from random states whose halos are nonzero and
+    differ between state and state_tmp, evolve() tracks the oracle, and
+    state and state_tmp match it whole, halos included."""
+    import pyminiweather_amd.solve as S
+    p = O.make_params(40, 20, "injection", cfl=1.5, max_speed=450.0)
+    orc = O.OracleSolver(p)
+    rng = np.random.default_rng(3)
+    scale = np.array([1e-3, 2.0, 2.0, 0.5])[:, None, None]
+    for key in ("state", "state_tmp"):
+        orc.f[key] += scale * rng.standard_normal(orc.f[key].shape)
+    f, mesh = gpu_fields(p, orc.f)
+    f.state_tmp.copy_(torch.from_numpy(orc.f["state_tmp"]))
+    for _ in range(3):
+        S.evolve(p, f, mesh, dt=p["dt"])
+    orc.run(3)
+    assert_fields_close(host(f.state), orc.f["state"], TOL_STEP, "injection state")
+    assert_fields_close(host(f.state_tmp), orc.f["state_tmp"], TOL_STEP,
+                        "injection state_tmp")
+
+
 def _evolve_ex(p, f, nsteps, tuning, flag=0):
     import ctypes
     import pyminiweather_amd.solve as S
