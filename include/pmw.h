@@ -343,6 +343,21 @@ int pmw_compute_stats(const double* state, const double* hy_dens_cell,
                       double dx, double dz, double* workspace,
                       long long workspace_len, double* out2, void* stream);
 
+/* pmw_compute_stats without a workspace, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). No counterpart in the reference. The
+ * per-block partials go to a 16 KiB array in the library's own device memory
+ * (one per GPU, held until the process ends), which covers every grid size:
+ * pass 1 never uses more than 1024 blocks. Allocates nothing; the stream
+ * (NULL = null stream) must belong to the current device, else PMW_EINVAL.
+ * Same kernels and summation order as pmw_compute_stats, so out2 is bit for
+ * bit the same. All calls on one GPU share the array: calls that overlap on
+ * one GPU (different streams, or host threads) race and must be serialised
+ * by the caller. */
+int pmw_compute_stats_noalloc(const double* state, const double* hy_dens_cell,
+                              const double* hy_dens_theta_cell, int nx, int nz,
+                              double dx, double dz, double* out2, void* stream);
+
 /* ---- initial conditions ------------------------------------------------ */
 
 /* Replaces init (pyminiweather/ics/initial.py:11-105) with the quadrature

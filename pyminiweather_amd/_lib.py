@@ -67,6 +67,7 @@ SIGNATURES = {
                               _D, _I, _I, _IP, _P]),
     "pmw_stats_workspace_size": (_LL, [_I, _I]),
     "pmw_compute_stats": (_I, [_P, _P, _P, _I, _I, _D, _D, _P, _LL, _P, _P]),
+    "pmw_compute_stats_noalloc": (_I, [_P, _P, _P, _I, _I, _D, _D, _P, _P]),
     "pmw_init": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P, _I, _P]),
     "pmw_hydro_pressure": (_I, [_P, _P, _I, _P]),
 }

@@ -9,6 +9,12 @@
 // For a given grid size the summation order is fixed, so results are
 // reproducible run to run (they differ from NumPy's pairwise order only in
 // rounding).
+//
+// pmw_compute_stats_noalloc runs the same two kernels with the partials in
+// `scratch`, a file-scope device array, so that it needs no workspace and
+// allocates nothing.
+#include <mutex>
+
 #include "pmw_common.h"
 
 using namespace pmw;
@@ -102,6 +108,55 @@ int stats_blocks(int nx, int nz) {
   return nz < kSMaxBlocks ? nz : kSMaxBlocks;  // one or more rows per block
 }
 
+// Partials of pmw_compute_stats_noalloc: one (mass, energy) pair per block of
+// pass 1, which never has more than kSMaxBlocks blocks, so every grid fits.
+constexpr int kScratchLen = 2 * kSMaxBlocks;  // 16 KiB per GPU
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+// Each GPU holds its own copy at its own address; host code reaches it only
+// through the address hipGetSymbolAddress returns on that GPU (the name
+// itself is a host-side stand-in, not a device address).
+static __device__ __align__(256) double scratch[kScratchLen];
+
+namespace {
+
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device. Returns PMW_OK, PMW_EINVAL or a hipError_t.
+int scratch_on_current_device(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= kMaxDevices) return PMW_EINVAL;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    if (!p) return PMW_EINVAL;
+    g_scratch_dev[dev] = (double*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return PMW_OK;
+}
+
+int launch_stats(const double* state, const double* hy_dens_cell,
+                 const double* hy_dens_theta_cell, int nx, int nz, double dx,
+                 double dz, double* part, double* out2, hipStream_t st) {
+  const int nb = stats_blocks(nx, nz);
+  Grid g(nx, nz);
+  k_stats_partial<<<nb, kSThreads, 0, st>>>(state, hy_dens_cell, hy_dens_theta_cell,
+                                            g, part);
+  PMW_CHECK_LAUNCH();
+  k_stats_final<<<1, kSThreads, 0, st>>>(part, nb, dx, dz, out2);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -120,14 +175,33 @@ int pmw_compute_stats(const double* state, const double* hy_dens_cell,
     return PMW_EINVAL;
   const int nb = stats_blocks(nx, nz);
   if (workspace_len < 2LL * nb) return PMW_ESPACE;
+  return launch_stats(state, hy_dens_cell, hy_dens_theta_cell, nx, nz, dx, dz,
+                      workspace, out2, (hipStream_t)stream);
+}
+
+int pmw_compute_stats_noalloc(const double* state, const double* hy_dens_cell,
+                              const double* hy_dens_theta_cell, int nx, int nz,
+                              double dx, double dz, double* out2, void* stream) {
+  if (nx < 1 || nz < 1 || !state || !hy_dens_cell || !hy_dens_theta_cell || !out2)
+    return PMW_EINVAL;
+  const int nb = stats_blocks(nx, nz);
+  if (2LL * nb > kScratchLen) return PMW_ESPACE;  // cannot happen: nb <= kSMaxBlocks
   hipStream_t st = (hipStream_t)stream;
-  Grid g(nx, nz);
-  k_stats_partial<<<nb, kSThreads, 0, st>>>(state, hy_dens_cell, hy_dens_theta_cell,
-                                            g, workspace);
-  PMW_CHECK_LAUNCH();
-  k_stats_final<<<1, kSThreads, 0, st>>>(workspace, nb, dx, dz, out2);
-  PMW_CHECK_LAUNCH();
-  return PMW_OK;
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (st) {  // the partials must live on the GPU the stream runs on
+    hipDevice_t sdev = -1;
+    e = hipStreamGetDevice(st, &sdev);
+    if (e != hipSuccess) return (int)e;
+    if ((int)sdev != dev) return PMW_EINVAL;
+  }
+  double* part = nullptr;
+  const int rc = scratch_on_current_device(&part);
+  if (rc != PMW_OK) return rc;
+  // every partial is written by pass 1 before pass 2 reads it: nothing to clear
+  return launch_stats(state, hy_dens_cell, hy_dens_theta_cell, nx, nz, dx, dz, part,
+                      out2, st);
 }
 
 }  // extern "C"
