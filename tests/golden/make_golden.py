@@ -17,6 +17,11 @@ It pins what no closed form or invariant can pin:
 * ``evolve`` (pyminiweather/solve/step.py:85-143) trajectories, Z-first on the
   first call with the module-global toggle (step.py:18, 103-109, 143).
 * ``compute_stats`` (pyminiweather/post/stats.py:8-35) on those states.
+* ``compute_solution_variables`` (pyminiweather/post/stats.py:38-69) on
+  every evolve checkpoint, and the command-line driver's output loop
+  (pyminiweather/__main__.py:205-247: warm-ups, then the state and the
+  solution variables before every ``output_freq``-th step) for three short
+  runs (``cli_*``).
 
 Output: ``tests/golden/*.npz`` (compressed float64 arrays + scalars). The
 thermal 100x50 ``init`` output is ``state0`` of its evolve trajectory.
@@ -49,7 +54,7 @@ def main(ref_root):
     from pyminiweather.data import initialize_fields
     from pyminiweather.ics import Directions, init
     from pyminiweather.mesh import MeshData
-    from pyminiweather.post import compute_stats
+    from pyminiweather.post import compute_solution_variables, compute_stats
     from pyminiweather.solve import evolve
 
     out_dir = os.path.dirname(os.path.abspath(__file__))
@@ -125,6 +130,7 @@ def main(ref_root):
             evolve(params, fields, mesh, dt=params["dt"])
             if istep in checkpoints:
                 rec[f"state_{istep}"] = fields.state.copy()
+                rec[f"svars_{istep}"] = np.array(compute_solution_variables(params, fields))
                 if istep in keep_tmp:
                     rec[f"state_tmp_{istep}"] = fields.state_tmp.copy()
                 m, e = compute_stats(params, fields)
@@ -144,6 +150,30 @@ def main(ref_root):
          **trajectory("injection", 40, 20, {1, 10}, {1, 10}))
     save("evolve_thermal_100x50",
          **trajectory("thermal", 100, 50, {1, 2, 10, 1500}, {1}))
+
+    # ---- the driver's output loop (__main__.py:205-247) ---------------------
+    def cli_run(ic, nx, nz, nsteps, output_freq, nwarmups):
+        # the driver's defaults: --cfl 1.0 --max-speed 500 (__main__.py:109-122)
+        step_mod.reverse_direction = False
+        params, fields, mesh = setup(ic, nx, nz, cfl=1.0, max_speed=500.0)
+        m0, e0 = compute_stats(params, fields)
+        for _ in range(nwarmups):
+            evolve(params, fields, mesh, dt=params["dt"])
+        states, svars, steps = [], [], []
+        for istep in range(nsteps):
+            if output_freq > 0 and (istep + 1) % output_freq == 0:
+                states.append(fields.state.copy())
+                svars.append(np.array(compute_solution_variables(params, fields)))
+                steps.append(istep)
+            evolve(params, fields, mesh, dt=params["dt"])
+        m1, e1 = compute_stats(params, fields)
+        return dict(states=np.stack(states), svars=np.stack(svars), steps=np.array(steps),
+                    mass0=float(m0), energy0=float(e0), mass1=float(m1), energy1=float(e1),
+                    nsteps=nsteps, output_freq=output_freq, nwarmups=nwarmups,
+                    dt=float(params["dt"]))
+
+    for ic, warmups in (("collision", 0), ("injection", 0), ("density-current", 2)):
+        save(f"cli_{ic}_40x20", **cli_run(ic, 40, 20, 5, 2, warmups))
 
 
 if __name__ == "__main__":

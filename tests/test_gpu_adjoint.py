@@ -348,6 +348,67 @@ def _thermal_400x200_misfit():
     return solver, theta0, target, loss, grad
 
 
+def _autograd_400x200(solver, theta0, residual_of, steps=400):
+    """FP64 torch autograd of L = 1/2 ||(rho*theta)'(T) - target||^2 through
+    oracle/torch_ref.evolve, one torch.utils.checkpoint per timestep (each
+    step's input kept, its graph recomputed in the backward pass).
+    `residual_of` maps the final (rho*theta)' to a list of residuals; returns
+    the gradient w.r.t. theta' for each."""
+    from torch.utils.checkpoint import checkpoint
+    from oracle import torch_ref as R
+    p, f = solver.p, solver.f
+    fd = _fdict(f)
+    nz, nx = f.nz, f.nx
+
+    def one_step(u, rev):
+        return R.evolve(u, p, fd, 1, reverse_direction=rev)
+
+    th = theta0.clone().requires_grad_(True)
+    w = f.hy_dens_cell[2:nz + 2][:, None]
+    u = torch.cat([torch.zeros((3, nz, nx), dtype=torch.float64, device="cuda"),
+                   (w * th)[None]], dim=0)
+    rev = False
+    for _ in range(steps):
+        u = checkpoint(one_step, u, rev, use_reentrant=False)
+        rev = not rev
+    resids = residual_of(u[3])
+    grads = []
+    for j, r in enumerate(resids):
+        (g,) = torch.autograd.grad(0.5 * torch.sum(r * r), th,
+                                   retain_graph=j + 1 < len(resids))
+        grads.append(g)
+    return grads
+
+
+def test_400x200_gradient_matches_torch_autograd():
+    """The 400x200, 400-step gradient (sqrt(N) = 20-step checkpoints) of the
+    hand-written adjoint against FP64 torch autograd through the restated
+    step (oracle/torch_ref.py, itself pinned to the NumPy oracle) to 1e-11
+    relative L2 over the whole gradient. The same autograd forward also
+    checks the 64-member batched assimilation's first-iteration gradient of
+    members 0 and 1 (targets from seeds 2 and 3): both ends of the batch
+    agree with autograd to the same bar."""
+    from pyminiweather_amd.adjoint import AdjointSolver
+    solver, theta0, target, loss, grad = _thermal_400x200_misfit()
+    M = 64
+    batched = AdjointSolver(solver.p, solver.f, 400, members=M)
+    noise = np.stack([np.random.default_rng(2 + m).normal(0.0, 0.1, (200, 400))
+                      for m in range(M)])
+    theta_m = theta0.expand(M, 200, 400).clone()
+    targets = batched.final_rhot(theta_m + torch.from_numpy(noise).cuda())
+    _, grads_m = batched.loss_and_grad(theta_m, targets)
+    refs = _autograd_400x200(solver, theta0,
+                             lambda fin: [fin - target, fin - targets[0], fin - targets[1]])
+    report = []
+    for name, got, ref in (("single", grad, refs[0]), ("member 0", grads_m[0], refs[1]),
+                           ("member 1", grads_m[1], refs[2])):
+        r = rel_l2(got.cpu().numpy(), ref.cpu().numpy())
+        report.append(f"{name}: rel L2 {r:.3e}")
+    print("; ".join(report), flush=True)
+    for line in report:
+        assert float(line.split()[-1]) <= 1e-11, report
+
+
 def _central_fd(solver, theta0, target, k, i, eps):
     e = torch.zeros_like(theta0)
     e[k, i] = eps
@@ -396,6 +457,48 @@ def test_400x200_gradient_matches_central_differences():
         if eps == 1e-3:
             assert rel <= 1e-6, report
     print("; ".join(report))
+
+
+def test_fd_floor_is_the_reference_arithmetics_own():
+    """Central differences computed entirely by the NumPy oracle (the
+    reference's own arithmetic; tests/golden/make_fd_100x50.py) on the
+    thermal misfit at 100x50, 400 steps, against the HIP adjoint's gradient
+    for the same theta0 and target.
+
+    The oracle's quotients show the same 1/eps round-off law as the HIP
+    path's: over the 16 cells they agree with the adjoint to ~1e-7 at
+    eps = 1e-4 and ~1e-8 at 1e-3, and move away as 1/eps below that,
+    exactly as the spread of the oracle's own loss under 1e-13 perturbations
+    (sigma_L) predicts. So 1e-6 relative at eps = 1e-6 is out of reach of
+    FP64 evaluation of this loss, whoever evaluates it: each quotient is held
+    to 6 sigma of that noise (plus 1e-6 relative), and the eps = 1e-3
+    quotients to 1e-6 relative in L2."""
+    from pyminiweather_amd.adjoint import AdjointSolver
+    from conftest import load_golden
+    fx = load_golden("fd_thermal_100x50")
+    p = O.make_params(100, 50, "thermal", cfl=1.5, max_speed=450.0)
+    f = _fields(p)
+    solver = AdjointSolver(p, f, 400)
+    theta0 = torch.from_numpy(fx["theta0"]).cuda()
+    target = torch.from_numpy(fx["target"]).cuda()
+    loss, grad = solver.loss_and_grad(theta0, target)
+    assert abs(loss - float(fx["loss"])) <= 1e-11 * float(fx["loss"])
+    g = np.array([float(grad[k, i]) for k, i in fx["cells"]])
+    sigma_l = float(np.sqrt(np.mean(np.square(fx["jitter"]))))
+    assert 0.0 < sigma_l < 1e-10 * float(fx["loss"])
+    rels = []
+    for a, eps in enumerate(fx["eps"]):
+        fd = fx["fd"][a]
+        sigma_fd = math.sqrt(2.0) * sigma_l / (2 * eps)
+        err = np.abs(fd - g)
+        bad = np.nonzero(err > 6 * sigma_fd + 1e-6 * np.abs(g))[0]
+        assert bad.size == 0, [(fx["cells"][j], fd[j], g[j], sigma_fd) for j in bad]
+        rels.append(float(np.linalg.norm(fd - g) / np.linalg.norm(g)))
+        print(f"eps={eps:g}: oracle FD vs HIP adjoint rel L2 {rels[-1]:.3e}, "
+              f"round-off sigma {sigma_fd / np.linalg.norm(g) * 4:.2e} relative", flush=True)
+    assert rels[-1] <= 1e-6
+    # round-off law: 1000x smaller eps, a disagreement >= 30x larger
+    assert rels[0] >= 30 * rels[-1]
 
 
 def test_64_member_assimilation_reduces_every_loss():

@@ -118,6 +118,58 @@ def test_members_match_oracle():
             assert r <= tol, f"member {q} var {v}: {r:.3e} > {tol:.2e}"
 
 
+def test_config_size_members_match_oracle_after_300_steps():
+    """One member of each grid size of the seeded 256-member ensemble (the
+    first 200x100, 400x200 and 800x400 of the seed-0 draw), 300 steps in one
+    batched launch per direction, against the oracle's 300-step runs
+    (tests/golden/ensemble_members_300.npz; the oracle reproduces the
+    reference bit for bit). Members start from the oracle's initial state and
+    profiles (hy_pressure_int re-derived on the device, as each platform's
+    init does). Each variable is held to max(1e-12, 4 x the oracle's own
+    drift under 1% last-bit pow noise) in relative L2 over the stored
+    samples (every 2nd/4th/8th row and column); the log names the binding
+    term. Mass and energy agree to 1e-12."""
+    from pyminiweather_amd import _lib
+    from pyminiweather_amd.ensemble import Ensemble, mixed_thermal_members
+    from pyminiweather_amd.post import compute_stats
+    from conftest import load_golden
+    fx = load_golden("ensemble_members_300")
+    params, bubbles = mixed_thermal_members(256, seed=0)
+    tags = ("200x100", "400x200", "800x400")
+    idx = [int(fx[f"{t}_index"]) for t in tags]
+    for t, q in zip(tags, idx):
+        assert f"{params[q]['nx']}x{params[q]['nz']}" == t
+        np.testing.assert_array_equal(np.array(bubbles[q]), fx[f"{t}_bubble"])
+    ens = Ensemble([params[q] for q in idx], [bubbles[q] for q in idx])
+    for j, q in enumerate(idx):
+        orc = O.init_state(params[q], bubbles[q])
+        f = ens.fields[j]
+        f.state.copy_(torch.from_numpy(orc["state"]))
+        f.state_tmp.copy_(f.state)
+        for k in ("hy_dens_cell", "hy_dens_theta_cell", "hy_dens_int", "hy_dens_theta_int"):
+            getattr(f, k).copy_(torch.from_numpy(orc[k]))
+        _lib.call("pmw_hydro_pressure", _lib.ptr(f.hy_dens_theta_int),
+                  _lib.ptr(f.hy_pressure_int), f.nz, _lib.stream_of())
+    ens.evolve(300)
+    for j, t in enumerate(tags):
+        f = ens.fields[j]
+        st = int(fx[f"{t}_stride"])
+        got = _host(f.state)[:, 2:-2, 2:-2][:, ::st, ::st]
+        ref = fx[f"{t}_sample"]
+        line = []
+        for v in range(4):
+            r = rel_l2(got[v], ref[v])
+            band = 4 * float(fx[f"{t}_noise_rel"][v])
+            tol = max(1e-12, band)
+            line.append(f"var {v} {r:.2e} (bar {tol:.1e}: "
+                        f"{'1e-12' if tol == 1e-12 else '4x oracle pow drift'})")
+            assert r <= tol, f"{t} var {v}: {r:.3e} > {tol:.2e}"
+        m, e = compute_stats(ens.params[j], f)
+        assert abs(m - fx[f"{t}_mass"]) <= 1e-12 * abs(fx[f"{t}_mass"])
+        assert abs(e - fx[f"{t}_energy"]) <= 1e-12 * abs(fx[f"{t}_energy"])
+        print(f"{t} member {idx[j]} after 300 steps: " + "; ".join(line), flush=True)
+
+
 @pytest.mark.parametrize("pairs", [1, 3])
 def test_graph_replays_match_eager(pairs):
     """Each replay of a captured 2- or 6-step graph, with eager work and host

@@ -470,6 +470,50 @@ def test_full_size_init_and_trajectory(big):
                         "2000x1000 20 steps tmp")
 
 
+@pytest.mark.parametrize("hydro", ["device", "reference"])
+def test_full_size_500_steps_match_fixture(big, hydro):
+    """The 2000 x 1000 collision workload at its stated length: 500 steps of
+    run() from the oracle's initial state against the oracle's own 500-step
+    trajectory (tests/golden/collision_2000x1000_500.npz; the oracle is
+    bit-identical to the reference on every reference-written golden). At
+    steps 100, 250 and 500 each variable is held, in relative L2 over every
+    16th row and column, to max(1e-12, 4 x the oracle's own drift when 1%
+    of its pow results move by one ulp) -- the log names the binding term --
+    and the whole-interior L2 norm, mass and energy to 1e-12. hydro
+    "device": hy_pressure_int re-derived with the device pow (as each
+    platform's init does); "reference": the oracle's (NumPy's) own profile,
+    as a caller handing over reference-built hydrostatics would."""
+    import pyminiweather_amd.solve as S
+    from pyminiweather_amd.post import compute_stats
+    fx = load_golden("collision_2000x1000_500")
+    S.reverse_direction = False
+    p, ref = big
+    assert p["dt"] == float(fx["dt"])
+    f, mesh = gpu_fields(p, ref)
+    if hydro == "reference":
+        f.hy_pressure_int.copy_(torch.from_numpy(ref["hy_pressure_int"]))
+    rows, cols = fx["rows"], fx["cols"]
+    done = 0
+    for n in (int(x) for x in fx["steps"]):
+        S.run(p, f, mesh, n - done)
+        done = n
+        s = interior(host(f.state))
+        got = s[:, rows][:, :, cols]
+        line = []
+        for v in range(4):
+            r = rel_l2(got[v], fx[f"sample_{n}"][v])
+            tol = max(TOL_STEP, 4 * float(fx[f"noise_rel_{n}"][v]))
+            line.append(f"var {v} {r:.2e} (bar {tol:.1e}: "
+                        f"{'1e-12' if tol == TOL_STEP else '4x oracle pow drift'})")
+            assert r <= tol, f"step {n} var {v}: {r:.3e} > {tol:.2e}"
+            nrm = float(np.linalg.norm(s[v]))
+            assert abs(nrm - fx[f"norm_{n}"][v]) <= TOL_STEP * fx[f"norm_{n}"][v]
+        m, e = compute_stats(p, f)
+        assert abs(m - float(fx[f"mass_{n}"])) <= TOL_STEP * abs(float(fx[f"mass_{n}"]))
+        assert abs(e - float(fx[f"energy_{n}"])) <= TOL_STEP * abs(float(fx[f"energy_{n}"]))
+        print(f"2000x1000 ({hydro} hy_pressure_int) step {n}: " + "; ".join(line), flush=True)
+
+
 def test_full_size_invariants(big):
     """Size-independent properties at 2000 x 1000 after 60 steps: mass
     conserved, mirror symmetry of the centred collision."""

@@ -73,7 +73,38 @@ def test_evolve_matches_reference(ic, nx, nz, steps):
             np.testing.assert_array_equal(S.f["state_tmp"], g[f"state_tmp_{st}"])
         m, e = S.stats()
         assert m == g[f"mass_{st}"] and e == g[f"energy_{st}"]
+        # compute_solution_variables (post/stats.py:38-69)
+        np.testing.assert_array_equal(O.solution_variables(p, S.f, S.f["state"]),
+                                      g[f"svars_{st}"])
     assert S.reverse_direction == bool(g["reverse_direction_end"])
+
+
+@pytest.mark.parametrize("ic", ["collision", "injection", "density-current"])
+def test_driver_output_loop_matches_reference(ic):
+    """The driver's output loop (pyminiweather/__main__.py:205-247): the
+    states and solution variables written before every output_freq-th step
+    after the warm-ups, and the start/end stats, as the reference's own
+    functions produced them."""
+    g = load_golden(f"cli_{ic}_40x20")
+    p = O.make_params(40, 20, ic)   # the driver's defaults: cfl 1.0, max_speed 500
+    assert p["dt"] == float(g["dt"])
+    S = O.OracleSolver(p)
+    m0, e0 = S.stats()
+    assert m0 == g["mass0"] and e0 == g["energy0"]
+    S.run(int(g["nwarmups"]))
+    k = 0
+    freq = int(g["output_freq"])
+    for istep in range(int(g["nsteps"])):
+        if (istep + 1) % freq == 0:
+            assert g["steps"][k] == istep
+            np.testing.assert_array_equal(S.f["state"], g["states"][k])
+            np.testing.assert_array_equal(O.solution_variables(p, S.f, S.f["state"]),
+                                          g["svars"][k])
+            k += 1
+        S.timestep()
+    assert k == len(g["steps"])
+    m1, e1 = S.stats()
+    assert m1 == g["mass1"] and e1 == g["energy1"]
 
 
 def test_scipy_variant_matches_slice_variant_over_steps():
