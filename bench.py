@@ -45,6 +45,10 @@ def workload_params(nx=2000, nz=1000):
 
 
 PROFILE_SUMMARY = os.path.join(REPO, "profiles", "fused_direction_kernels_collision_2000x1000.txt")
+# FP64 instruction issue: each of the 1024 SIMDs (256 CUs x 4) issues one wave64
+# FP64 VALU instruction per 4 cycles (16 lanes per cycle) at 2.4 GHz -- the
+# 78.6 TFLOP/s FP64 peak counted in instructions instead of flops.
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4   # wave64 VALU instructions per second
 
 
 def traffic_from_profile(kernel):
@@ -59,6 +63,21 @@ def traffic_from_profile(kernel):
                 m = pat.match(line)
                 if m and m.group(1) == kernel:
                     return float(m.group(2))
+    except OSError:
+        pass
+    return None
+
+
+def valu_from_profile(kernel):
+    """SQ_INSTS_VALU (wave64 VALU instructions) per launch of a kernel from
+    the committed rocprofv3 --pmc summary, or None."""
+    try:
+        with open(PROFILE_SUMMARY) as fh:
+            for line in fh:
+                if line.startswith("valu_insts_per_launch "):
+                    name, n = line[len("valu_insts_per_launch "):].rsplit(" ", 1)
+                    if name == kernel:
+                        return float(n)
     except OSError:
         pass
     return None
@@ -424,9 +443,15 @@ def main():
         tr = traffic_from_profile(prefixes[d])
         if tr:
             kern[names[d]]["GBps_measured_traffic"] = round(tr / avg_s / 1e9, 1)
+            kern[names[d]]["traffic_frac"] = round(tr / avg_s / 1e9 / HBM_PEAK_GBS, 4)
+        vi = valu_from_profile(prefixes[d])
+        if vi:
+            kern[names[d]]["valu_issue_frac"] = round(vi / avg_s / VALU_ISSUE_PEAK, 4)
     dom = 0 if kms[0] >= kms[1] else 1
     dom_avg_s = kms[dom] / launches / 1e3
     achieved = BYTES_PER_CELL_LAUNCH * cells / dom_avg_s / 1e9
+    dom_traffic = traffic_from_profile(prefixes[dom])
+    dom_valu = valu_from_profile(prefixes[dom])
     m1, e1 = compute_stats(p, f)
     if not (abs(m1 - m0) <= 1e-12 * abs(m0)):
         raise RuntimeError(f"mass not conserved: {m0} -> {m1}")
@@ -447,10 +472,28 @@ def main():
         "config": {"workload": f"collision {nx}x{nz} FP64 forward, RK3 + x/z splitting",
                    "nx": nx, "nz": nz, "dt": p["dt"], "cfl": 1.5, "max_speed": 450.0,
                    "tuning": list(S.tuning_for(nx, nz))},
+        # achieved/frac: the reference algorithm's 256 B per cell and direction
+        # over the fused launch's time. The fused kernels move less (traffic:
+        # S read once, the result written), so the resource that binds them is
+        # FP64 instruction issue: compute.frac, from the counted VALU
+        # instructions per launch, against traffic_frac.
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profile(prefixes[dom]), "kernel": names[dom],
-                     "bytes_per_launch": BYTES_PER_CELL_LAUNCH * cells},
+                     "traffic": dom_traffic, "kernel": names[dom],
+                     "bytes_per_launch": BYTES_PER_CELL_LAUNCH * cells,
+                     "traffic_frac": (round(dom_traffic / dom_avg_s / 1e9 / HBM_PEAK_GBS, 4)
+                                      if dom_traffic else None),
+                     "compute": ({"resource": "FP64 VALU issue (1024 SIMDs, one wave64 "
+                                              "instruction per 4 cycles, 2.4 GHz)",
+                                  "valu_insts_per_launch": dom_valu,
+                                  "achieved": round(dom_valu / dom_avg_s / 1e9, 1),
+                                  "peak": VALU_ISSUE_PEAK / 1e9,
+                                  "unit": "G wave64 VALU instructions/s",
+                                  "frac": round(dom_valu / dom_avg_s / VALU_ISSUE_PEAK, 4)}
+                                 if dom_valu else None),
+                     "binding": ("fp64_valu_issue" if dom_valu and dom_traffic and
+                                 dom_valu / VALU_ISSUE_PEAK > dom_traffic / (HBM_PEAK_GBS * 1e9)
+                                 else "hbm")},
         "step_roofline_frac": round(value * BYTES_PER_CELL_STEP / 1e9 / HBM_PEAK_GBS, 4),
         "gpu_ms_per_step": gpu_s / args.steps * 1e3,
         "kernels": kern,

@@ -31,6 +31,18 @@ def test_traffic_of_the_default_kernels_is_in_the_committed_summary():
     assert bench.traffic_from_profile("k_dir_x") is None  # exact names only
 
 
+def test_valu_counts_of_the_default_kernels_are_in_the_committed_summary():
+    """Each default kernel has its counted VALU instructions per launch, and
+    the FP64 issue floor they imply lies below the kernel's traced average
+    duration (a floor above it would mean the counter or the unit is wrong)."""
+    for k in bench.fused_kernel_names((0, 512, 8128)):
+        n = bench.valu_from_profile(k)
+        assert n is not None and n > 1e6, k
+        floor_us = n / bench.VALU_ISSUE_PEAK * 1e6
+        assert 20.0 < floor_us < 60.0, (k, floor_us)
+    assert bench.VALU_ISSUE_PEAK == 256 * 4 * 2.4e9 / 4
+
+
 def test_roofline_units():
     assert bench.BYTES_PER_CELL_STEP == 2 * bench.BYTES_PER_CELL_LAUNCH == 512.0
     assert bench.FWD_ADJ_BYTES_PER_CELL_STEP == 512.0 + 512.0 + 768.0
