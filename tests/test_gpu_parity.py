@@ -288,27 +288,79 @@ def test_rk_stage_rejects_aliased_output():
 TRAJ = [("collision", 40, 20, [1, 2, 25]), ("gravity", 40, 20, [1, 10]),
         ("density-current", 40, 20, [1, 10]), ("injection", 40, 20, [1, 10]),
         ("thermal", 100, 50, [1, 2, 10, 1500])]
-# Per-variable trajectory tolerance. The gravity-wave IC is ill-conditioned
-# in rho', rho*w and (rho*theta)': their early signal (~1e-6) sits below the
-# last-bit noise of the full ~1e5 Pa pressure in the x flux, so flipping the
-# last bit of 1% of NumPy's own pow results moves them by ~5e-11..3e-10
-# relative (measured on the oracle by
-# test_oracle_known_answers.py::test_gravity_ic_pow_conditioning). Device pow
-# and glibc pow differ in the last bit on some inputs, so those three fields
-# are held to 1e-9 there; every other field and IC to 1e-12.
-TRAJ_TOL = {"gravity": (1e-9, TOL_STEP, 1e-9, 1e-9)}
+# Per-variable trajectory bar: max(1e-12, 4 x the reference arithmetic's own
+# drift when 1% of its pow results move by one ulp), measured here on the
+# oracle for the same IC, grid and checkpoints. The collision, density-current,
+# injection and thermal trajectories drift < 1e-12 that way, so they are held
+# to 1e-12. The gravity-wave IC is ill-conditioned in rho', rho*w and
+# (rho*theta)': their early signal (~1e-6) sits below the last-bit noise of
+# the full ~1e5 Pa pressure in the x flux, and the oracle's own drift is
+# 5e-11..3e-10 there (test_oracle_known_answers.py::
+# test_gravity_ic_pow_conditioning); device and glibc pow differ in the last
+# bit on some inputs, so those fields get the measured band. The test prints
+# which term binds.
+_DRIFT = {}
 
 
+def _oracle_pow_drift(p, steps, rate=0.01, seeds=(5, 6)):
+    """Relative L2 change per variable of the oracle's own trajectory at each
+    checkpoint in `steps` when the last bit of `rate` of its pow results is
+    flipped (the larger over two seeds); a variable that is identically zero
+    in the clean run gets 0 (the zero-field check applies there). Returns
+    {step: [4 floats]}, cached per grid, IC and checkpoints."""
+    key = (p["ic_type"], p["nx"], p["nz"], tuple(steps))
+    if key not in _DRIFT:
+        runs = [_pow_drift_once(p, steps, rate, sd) for sd in seeds]
+        _DRIFT[key] = {st: [max(r[st][v] for r in runs) for v in range(4)] for st in steps}
+    return _DRIFT[key]
+
+
+def _pow_drift_once(p, steps, rate, seed):
+    import oracle.numpy_ref as R
+    power = np.power
+    rng = np.random.default_rng(seed)
+
+    def noisy(a, b, *args, **kw):
+        r = power(a, b, *args, **kw)
+        if isinstance(r, np.ndarray) and r.size > 50:
+            flip = rng.random(r.shape) < rate
+            up = rng.random(r.shape) < 0.5
+            r = np.where(flip, np.nextafter(r, np.where(up, np.inf, -np.inf)), r)
+        return r
+    clean = O.OracleSolver(p)
+    pert = O.OracleSolver(p)
+    out = {}
+    done = 0
+    for st in steps:
+        clean.run(st - done)
+        R.np.power = noisy
+        try:
+            pert.run(st - done)
+        finally:
+            R.np.power = power
+        done = st
+        out[st] = [rel_l2(pert.f["state"][v], clean.f["state"][v])
+                   if np.any(clean.f["state"][v]) else 0.0 for v in range(4)]
+    return out
+
+
+@pytest.mark.parametrize("hydro", ["device", "reference"])
 @pytest.mark.parametrize("ic,nx,nz,steps", TRAJ)
-def test_evolve_matches_reference(ic, nx, nz, steps):
+def test_evolve_matches_reference(ic, nx, nz, steps, hydro):
     """From the reference's own initial state: evolve() step by step, then
-    compare state, state_tmp (halos included) and the stats."""
+    compare state, state_tmp (halos included) and the stats. hydro "device":
+    hy_pressure_int re-derived with the device pow, as each platform's init
+    does; "reference": the reference's own profile, as a caller handing over
+    reference-built hydrostatics would."""
     import pyminiweather_amd.solve as S
     from pyminiweather_amd.post import compute_stats
     g = load_golden(f"evolve_{ic}_{nx}x{nz}")
     p = O.make_params(nx, nz, ic, cfl=1.5, max_speed=450.0)
     assert p["dt"] == g["dt"]
     f, mesh = gpu_fields(p, g)
+    if hydro == "reference":
+        f.hy_pressure_int.copy_(torch.from_numpy(g["hy_pressure_int"]))
+    drift = _oracle_pow_drift(p, steps)
     m0, e0 = compute_stats(p, f)
     assert abs(m0 - g["mass0"]) <= 1e-13 * abs(g["mass0"])
     assert abs(e0 - g["energy0"]) <= 1e-13 * abs(g["energy0"])
@@ -321,8 +373,13 @@ def test_evolve_matches_reference(ic, nx, nz, steps):
                 S.evolve(p, f, mesh, dt=p["dt"])
         done = st
         s = host(f.state)
-        tol = TRAJ_TOL.get(ic, TOL_STEP)
-        assert_fields_close(s, g[f"state_{st}"], tol, f"{ic} step {st} state")
+        tol = [max(TOL_STEP, 4 * d) for d in drift[st]]
+        ref = g[f"state_{st}"]
+        print(f"{ic} ({hydro} hy_pressure_int) step {st}: " + "; ".join(
+            f"var {v} {rel_l2(s[v], ref[v]):.2e} (bar {tol[v]:.1e}: "
+            f"{'1e-12' if tol[v] == TOL_STEP else '4x oracle pow drift'})" for v in range(4)),
+            flush=True)
+        assert_fields_close(s, ref, tol, f"{ic} step {st} state")
         if f"state_tmp_{st}" in g:
             assert_fields_close(host(f.state_tmp), g[f"state_tmp_{st}"], tol,
                                 f"{ic} step {st} state_tmp")
