@@ -86,17 +86,35 @@ __device__ __forceinline__ void recon4(double s0, double s1, double s2,
   d3 = ((s0 * -1.0 + s1 * 3.0) + s2 * -3.0) + s3 * 1.0;
 }
 
+// The pressure power (rho*theta)^gamma of a face: POW_PLAIN evaluates
+// pow_pos; POW_FIRST also keeps its unrounded value in `ref` (the first RK
+// stage of a fused direction); POW_NEXT continues from `ref` (the later
+// stages, whose face values differ from the first stage's by a relative
+// ~1e-4 or less: pow_pos_near, with pow_pos for larger changes).
+enum PowMode { POW_PLAIN = 0, POW_FIRST = 1, POW_NEXT = 2 };
+
+__device__ __forceinline__ double face_power(double q, int mode, PowRef* ref) {
+  if (mode == POW_PLAIN) return pow_pos(q, kGamma);
+  double r;
+  if (mode == POW_NEXT && pow_pos_near_try(q, kGamma, *ref, r)) return r;
+  // first stage, or a continuation too close to a rounding boundary (about
+  // one argument in 1400): pow_pos; only the first stage keeps the reference
+  PowRef scratch;
+  return pow_pos_ref(q, kGamma, mode == POW_FIRST ? *ref : scratch);
+}
+
 // x-direction Euler flux with hyperviscosity (solve/interpolate.py:105-129).
 __device__ __forceinline__ void flux_x_face(const double val[4],
                                             const double d3[4], double hyr,
                                             double hyt, double hv,
-                                            double F[4]) {
+                                            double F[4], int pmode = POW_PLAIN,
+                                            PowRef* ref = nullptr) {
   const double rho = val[0] + hyr;
   const double ir = 1.0 / rho;
   const double u = div_rn(val[1], rho, ir);
   const double w = div_rn(val[2], rho, ir);
   const double t = div_rn(val[3] + hyt, rho, ir);
-  const double p = kC0 * pow_pos(rho * t, kGamma);
+  const double p = kC0 * face_power(rho * t, pmode, ref);
   const double ru = rho * u;
   F[0] = ru - hv * d3[0];
   F[1] = (rho * (u * u) + p) - hv * d3[1];
@@ -109,13 +127,14 @@ __device__ __forceinline__ void flux_x_face(const double val[4],
 __device__ __forceinline__ void flux_z_face(const double val[4], double d3[4],
                                             double hyr, double hyt,
                                             double hyp, double hv, bool wall,
-                                            double G[4]) {
+                                            double G[4], int pmode = POW_PLAIN,
+                                            PowRef* ref = nullptr) {
   const double rho = val[0] + hyr;
   const double ir = 1.0 / rho;
   const double u = div_rn(val[1], rho, ir);
   double w = div_rn(val[2], rho, ir);
   const double t = div_rn(val[3] + hyt, rho, ir);
-  const double p = kC0 * pow_pos(rho * t, kGamma) - hyp;
+  const double p = kC0 * face_power(rho * t, pmode, ref) - hyp;
   if (wall) {
     w = 0.0;
     d3[0] = 0.0;

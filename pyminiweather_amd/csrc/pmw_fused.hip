@@ -93,15 +93,18 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
   const double src = a.src_w ? a.src_w[srow + cw] : 0.0;
   const double id = 1.0 / a.d;
   __syncthreads();
+  // face j is computed by thread j in all three stages, which keeps its
+  // stage-1 pressure power for the later stages (POW_FIRST / POW_NEXT)
+  PowRef pref;
   // ---- stage 1: faces 2..NT-2, cells 2..NT-3, S -> s1
   {
-    const int j = t + 2;
-    if (j <= NT - 2) {
+    const int j = t;
+    if (j >= 2 && j <= NT - 2) {
       double v4[4], d3[4], F[4];
 #pragma unroll
       for (int v = 0; v < kNv; ++v)
         recon4(s0[v][j - 2], s0[v][j - 1], s0[v][j], s0[v][j + 1], v4[v], d3[v]);
-      flux_x_face(v4, d3, hyr, hyt, a.hv, F);
+      flux_x_face(v4, d3, hyr, hyt, a.hv, F, POW_FIRST, &pref);
 #pragma unroll
       for (int v = 0; v < kNv; ++v) fl[v][j] = F[v];
     }
@@ -122,13 +125,13 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
   __syncthreads();
   // ---- stage 2: faces 4..NT-4, cells 4..NT-5, s1 -> s1 (in place)
   {
-    const int j = t + 4;
-    if (j <= NT - 4) {
+    const int j = t;
+    if (j >= 4 && j <= NT - 4) {
       double v4[4], d3[4], F[4];
 #pragma unroll
       for (int v = 0; v < kNv; ++v)
         recon4(s1[v][j - 2], s1[v][j - 1], s1[v][j], s1[v][j + 1], v4[v], d3[v]);
-      flux_x_face(v4, d3, hyr, hyt, a.hv, F);
+      flux_x_face(v4, d3, hyr, hyt, a.hv, F, POW_NEXT, &pref);
 #pragma unroll
       for (int v = 0; v < kNv; ++v) fl[v][j] = F[v];
     }
@@ -157,13 +160,13 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
   __syncthreads();
   // ---- stage 3: faces 6..NT-6, cells 6..NT-7 -> out
   {
-    const int j = t + 6;
-    if (j <= NT - 6) {
+    const int j = t;
+    if (j >= 6 && j <= NT - 6) {
       double v4[4], d3[4], F[4];
 #pragma unroll
       for (int v = 0; v < kNv; ++v)
         recon4(s1[v][j - 2], s1[v][j - 1], s1[v][j], s1[v][j + 1], v4[v], d3[v]);
-      flux_x_face(v4, d3, hyr, hyt, a.hv, F);
+      flux_x_face(v4, d3, hyr, hyt, a.hv, F, POW_NEXT, &pref);
 #pragma unroll
       for (int v = 0; v < kNv; ++v) fl[v][j] = F[v];
     }
@@ -190,8 +193,8 @@ __device__ __forceinline__ void member_offset(DirArgs& a) {
 }
 
 template <int NT>
-__global__ __launch_bounds__(NT) void k_dir_x(DirArgs a, Grid g, int ntiles, int nreal,
-                                              int per_xcd) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_dir_x(
+    DirArgs a, Grid g, int ntiles, int nreal, int per_xcd) {
   constexpr int C = NT - 12;
   __shared__ XTileLds<NT> sm;
   const int logical = xcd_logical(blockIdx.x, per_xcd);
@@ -250,6 +253,7 @@ __device__ __forceinline__ void dir_xd_tile(const DirArgs& a, const Grid& g, int
   const double hyr = a.hy_cell[prow], hyt = a.hyt_cell[prow];
   const double src = (a.src_w && row_ok) ? a.src_w[(long long)r * nx + cw] : 0.0;
   const double id = 1.0 / a.d;
+  PowRef pref;  // this face's stage-1 pressure power, continued in stages 2 and 3
   __syncthreads();
 #pragma unroll
   for (int stage = 0; stage < 3; ++stage) {
@@ -260,7 +264,7 @@ __device__ __forceinline__ void dir_xd_tile(const DirArgs& a, const Grid& g, int
 #pragma unroll
       for (int v = 0; v < kNv; ++v)
         recon4(src_s[v][t - 2], src_s[v][t - 1], src_s[v][t], src_s[v][t + 1], v4[v], d3[v]);
-      flux_x_face(v4, d3, hyr, hyt, a.hv, F);
+      flux_x_face(v4, d3, hyr, hyt, a.hv, F, stage == 0 ? POW_FIRST : POW_NEXT, &pref);
 #pragma unroll
       for (int v = 0; v < kNv; ++v) fl[v][t] = F[v];
     }
@@ -295,7 +299,8 @@ __device__ __forceinline__ void dir_xd_tile(const DirArgs& a, const Grid& g, int
 }
 
 template <int NT>
-__global__ __launch_bounds__(NT) void k_dir_xd(DirArgs a, Grid g, int nreal, int per_xcd) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_dir_xd(
+    DirArgs a, Grid g, int nreal, int per_xcd) {
   __shared__ XTileLds<NT> sm;
   const int logical = xcd_logical(blockIdx.x, per_xcd);
   if (logical >= nreal) return;  // whole block, before any barrier
@@ -660,6 +665,8 @@ __device__ __forceinline__ void dir_zt_tile(const DirArgs& a, const Grid& g, int
 #pragma unroll
       for (int v = 0; v < kNv; ++v)
         recon4(sin[v][t - 2 * CW], sin[v][t - CW], sin[v][t], sin[v][t + CW], v4[v], d3[v]);
+      // the plain power: this tile's 72 VGPRs leave no room for the
+      // stage-1 reference at 6 waves per SIMD (results are the same)
       flux_z_face(v4, d3, a.hyr_int[k], a.hyt_int[k], a.hyp_int[k], a.hv, k == 0 || k == nz, G);
 #pragma unroll
       for (int v = 0; v < kNv; ++v) fl[v][t] = G[v];
@@ -799,6 +806,7 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
     }
   }
   const double id = 1.0 / a.d;
+  PowRef pref[2];  // each thread computes the faces at its two slots in every stage
   __syncthreads();
 #pragma unroll
   for (int stage = 0; stage < 3; ++stage) {
@@ -812,7 +820,7 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
         for (int v = 0; v < kNv; ++v)
           recon4(s[v][u - 2 * CW], s[v][u - CW], s[v][u], s[v][u + CW], v4[v], d3[v]);
         flux_z_face(v4, d3, a.hyr_int[k], a.hyt_int[k], a.hyp_int[k], a.hv, k == 0 || k == nz,
-                    G);
+                    G, stage == 0 ? POW_FIRST : POW_NEXT, &pref[c]);
 #pragma unroll
         for (int v = 0; v < kNv; ++v) fl[v][u] = G[v];
       }
@@ -954,6 +962,7 @@ __device__ __forceinline__ void dir_zt2d_tile(const DirArgs& a, const Grid& g, i
   }
   gdouble* const tmp_own = a.tmp_interior ? a.tmp : nullptr;
   const double id = 1.0 / a.d;
+  PowRef pref[2];  // each thread computes the faces at its two slots in every stage
   __syncthreads();
 #pragma unroll
   for (int stage = 0; stage < 3; ++stage) {
@@ -967,7 +976,7 @@ __device__ __forceinline__ void dir_zt2d_tile(const DirArgs& a, const Grid& g, i
         for (int v = 0; v < kNv; ++v)
           recon4(s[v][u - 2 * CW], s[v][u - CW], s[v][u], s[v][u + CW], v4[v], d3[v]);
         flux_z_face(v4, d3, a.hyr_int[k], a.hyt_int[k], a.hyp_int[k], a.hv, k == 0 || k == nz,
-                    G);
+                    G, stage == 0 ? POW_FIRST : POW_NEXT, &pref[c]);
 #pragma unroll
         for (int v = 0; v < kNv; ++v) fl[v][u] = G[v];
       }
@@ -1021,7 +1030,7 @@ __device__ __forceinline__ void dir_zt2d_tile(const DirArgs& a, const Grid& g, i
 }
 
 template <int CW, int NR>
-__global__ __launch_bounds__(CW * NR / 2) void k_dir_zt2d(DirArgs a, Grid g, int nreal,
+__global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_dir_zt2d(DirArgs a, Grid g, int nreal,
                                                           int per_xcd) {
   __shared__ ZTile2Lds<CW * NR> sm;
   const int logical = xcd_logical(blockIdx.x, per_xcd);
@@ -1031,8 +1040,8 @@ __global__ __launch_bounds__(CW * NR / 2) void k_dir_zt2d(DirArgs a, Grid g, int
 }
 
 template <int CW, int NR>
-__global__ __launch_bounds__(CW * NR / 2) void k_dir_zt2(DirArgs a, Grid g, int ncolgrp,
-                                                         int nreal, int per_xcd) {
+__global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_dir_zt2(
+    DirArgs a, Grid g, int ncolgrp, int nreal, int per_xcd) {
   __shared__ ZTile2Lds<CW * NR> sm;
   const int logical = xcd_logical(blockIdx.x, per_xcd);
   if (logical >= nreal) return;  // whole block, before any barrier
@@ -1106,7 +1115,7 @@ __device__ __forceinline__ int member_of(const MemberDev* m, int n, int item) {
 // step (0: state -> work, 1: work -> state), bit 1 = store state_tmp's
 // interior (the last direction of the call only).
 template <int NT>
-__global__ __launch_bounds__(NT) void k_ens_x(const MemberDev* __restrict__ m, int n, int pass,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_ens_x(const MemberDev* __restrict__ m, int n, int pass,
                                               int total, int per_xcd) {
   constexpr int C = NT - 12;
   __shared__ XTileLds<NT> sm;
@@ -1121,7 +1130,7 @@ __global__ __launch_bounds__(NT) void k_ens_x(const MemberDev* __restrict__ m, i
 }
 
 template <int NT>
-__global__ __launch_bounds__(NT) void k_ens_xd(const MemberDev* __restrict__ m, int n, int pass,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_ens_xd(const MemberDev* __restrict__ m, int n, int pass,
                                                int total, int per_xcd) {
   __shared__ XTileLds<NT> sm;
   const int item = xcd_logical(blockIdx.x, per_xcd);
@@ -1162,7 +1171,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 }
 
 template <int CW, int NR>
-__global__ __launch_bounds__(CW * NR / 2) void k_ens_zt2d(const MemberDev* __restrict__ m, int n,
+__global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_ens_zt2d(const MemberDev* __restrict__ m, int n,
                                                           int pass, int total, int per_xcd) {
   __shared__ ZTile2Lds<CW * NR> sm;
   const int item = xcd_logical(blockIdx.x, per_xcd);
@@ -1175,7 +1184,7 @@ __global__ __launch_bounds__(CW * NR / 2) void k_ens_zt2d(const MemberDev* __res
 }
 
 template <int CW, int NR>
-__global__ __launch_bounds__(CW * NR / 2) void k_ens_zt2(const MemberDev* __restrict__ m, int n,
+__global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_ens_zt2(const MemberDev* __restrict__ m, int n,
                                                          int pass, int total, int per_xcd) {
   __shared__ ZTile2Lds<CW * NR> sm;
   const int item = xcd_logical(blockIdx.x, per_xcd);

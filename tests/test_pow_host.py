@@ -50,6 +50,29 @@ def test_pow_gamma_minus_one(checker):
     assert fast_err <= 5e-15, fast_err
 
 
+@pytest.mark.parametrize("lo,hi", [(50.0, 2000.0), (1e-3, 1e5)])
+def test_pow_near_errors_stay_inside_its_guard(checker, lo, hi):
+    """pow_pos_near decides that its rounding equals pow_pos's when its value
+    before rounding is more than 2^-64 relative from a midpoint between
+    doubles; that is sound while the two values before rounding together
+    lie within that of the exact power. Measured maxima (200k arguments and
+    continuations up to 2^-10 away): each ~1e-20, their sum a third of it."""
+    out = subprocess.run([checker, "200000", str(lo), str(hi), GAMMA, "err", "9.7e-4"],
+                         capture_output=True, text=True, check=True).stdout.split()
+    pp_err, near_err = float(out[1]), float(out[2])
+    assert pp_err + near_err < 0.5 * 2.0 ** -64, (pp_err, near_err)
+
+
+@pytest.mark.parametrize("lo,hi", [(50.0, 2000.0), (1e-3, 1e5)])
+def test_pow_near_equals_pow_pos(checker, lo, hi):
+    """The fused kernels' stages 2 and 3 continue each face's stage-1 power
+    (pow_pos_near): bit for bit pow_pos on 4e6 continuations (and on 2e9 in
+    a longer run of the same check over [50, 2000])."""
+    out = subprocess.run([checker, "4000000", str(lo), str(hi), GAMMA, "q", "9.7e-4"],
+                         capture_output=True, text=True).stdout.split()
+    assert int(out[1]) == 0, out
+
+
 def test_tables_regenerate_identically():
     """pmw_pow_tables.h is what gen_pow_tables.py writes."""
     r = subprocess.run(["python3", os.path.join(CSRC, "gen_pow_tables.py")],
