@@ -739,12 +739,21 @@ struct ZTile2Lds {
   double fl[kNv][NC];
 };
 
+// The same with the tile rows' face hydrostatics (rho, rho*theta, p at the
+// lower face of each tile row), read from HBM once per tile instead of once
+// per stage and cell; all CW columns of a row read one LDS word (broadcast).
+template <int CW, int NR>
+struct ZTile2HLds : ZTile2Lds<CW * NR> {
+  double hz[3][NR];
+};
+
 template <int CW, int NR>
 __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, int c0, int r0,
-                                             ZTile2Lds<CW * NR>& sm, bool corners) {
+                                             ZTile2HLds<CW, NR>& sm, bool corners) {
   constexpr int NC = CW * NR, NT = NC / 2, H = 6, C = NR - 2 * H;
   auto& s = sm.s;
   auto& fl = sm.fl;
+  auto& hz = sm.hz;
   const int nx = g.nx, nz = g.nz, t = threadIdx.x;
   const int col = t % CW;
   const int i = c0 + col;
@@ -754,7 +763,15 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
   const int plo = kHs, phi = nz + kHs - 1;
   const bool own_lo = r0 == 0, own_hi = r0 <= nz - 1 && nz - 1 < r0 + C;
   gdouble* const tmp_own = a.tmp_interior ? a.tmp : nullptr;
-  int jj[2], kk[2], sl[2], es[2];
+  static_assert(NT >= NR, "one thread per tile row fills the face hydrostatics");
+  if (t < NR) {
+    const int k = r0 - H + t;
+    const bool f = k >= 0 && k <= nz;
+    hz[0][t] = f ? a.hyr_int[k] : 0.0;
+    hz[1][t] = f ? a.hyt_int[k] : 0.0;
+    hz[2][t] = f ? a.hyp_int[k] : 0.0;
+  }
+  int jj[2], kk[2], sl[2];
   bool wl[2], wh[2], own[2];
   double S[2][4], src[2];
 #pragma unroll
@@ -765,7 +782,6 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
     wl[c] = kk[c] == -2 || kk[c] == -1;
     wh[c] = kk[c] == nz || kk[c] == nz + 1;
     own[c] = colok && kk[c] >= r0 && kk[c] < r0 + C && kk[c] < nz;
-    es[c] = wl[c] ? sl[c] - kk[c] * CW : sl[c] - (kk[c] - (nz - 1)) * CW;
     double q[4] = {0.0, 0.0, 0.0, 0.0};
     if (colok) {
       if (kk[c] >= 0 && kk[c] < nz) {
@@ -819,7 +835,7 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
 #pragma unroll
         for (int v = 0; v < kNv; ++v)
           recon4(s[v][u - 2 * CW], s[v][u - CW], s[v][u], s[v][u + CW], v4[v], d3[v]);
-        flux_z_face(v4, d3, a.hyr_int[k], a.hyt_int[k], a.hyp_int[k], a.hv, k == 0 || k == nz,
+        flux_z_face(v4, d3, hz[0][j], hz[1][j], hz[2][j], a.hv, k == 0 || k == nz,
                     G, stage == 0 ? POW_FIRST : POW_NEXT, &pref[c]);
 #pragma unroll
         for (int v = 0; v < kNv; ++v) fl[v][u] = G[v];
@@ -842,6 +858,30 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
         if (stage < 2) {
 #pragma unroll
           for (int v = 0; v < kNv; ++v) s[v][u] = q[v];  // in place: faces are done
+          // The wall halo rows of the new stage state, written by the edge
+          // row's thread: no thread reads them in this phase (every update
+          // reads its own cell and faces only), and a tile holding a wall
+          // has its edge row in the window, updated at every stage.
+#pragma unroll
+          for (int side = 0; side < 2; ++side) {
+            if (k == (side == 0 ? 0 : nz - 1)) {
+              const int pe = side == 0 ? plo : phi;
+              const int step = side == 0 ? -CW : CW;
+#pragma unroll
+              for (int h = 1; h <= 2; ++h) {
+                const int kh = side == 0 ? -h : nz - 1 + h;  // the halo row's k
+                double w[4];
+#pragma unroll
+                for (int v = 0; v < kNv; ++v) w[v] = wall_halo(v, q[v], hy[pe], hy[kh + kHs]);
+#pragma unroll
+                for (int v = 0; v < kNv; ++v) s[v][u + h * step] = w[v];
+                if (stage == 1 && (side == 0 ? own_lo : own_hi)) {  // set_bc_z of stage 3
+#pragma unroll
+                  for (int v = 0; v < kNv; ++v) a.tmp[g.at(v, kh + kHs, pc)] = w[v];
+                }
+              }
+            }
+          }
         }
         if (own[c]) {
           gdouble* dst = stage == 0 ? a.s1 : stage == 1 ? tmp_own : a.out;
@@ -853,22 +893,6 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
       }
     }
     if (stage == 2) break;
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      if (colok && (wl[c] || wh[c])) {
-        const int pe = wl[c] ? plo : phi;
-        double q[4];
-#pragma unroll
-        for (int v = 0; v < kNv; ++v) q[v] = wall_halo(v, s[v][es[c]], hy[pe], hy[kk[c] + kHs]);
-#pragma unroll
-        for (int v = 0; v < kNv; ++v) s[v][sl[c]] = q[v];
-        if (stage == 1 && ((wl[c] && own_lo) || (wh[c] && own_hi))) {  // set_bc_z of stage 3
-#pragma unroll
-          for (int v = 0; v < kNv; ++v) a.tmp[g.at(v, kk[c] + kHs, pc)] = q[v];
-        }
-      }
-    }
     __syncthreads();
   }
 }
@@ -1042,7 +1066,7 @@ __global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))
 template <int CW, int NR>
 __global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_dir_zt2(
     DirArgs a, Grid g, int ncolgrp, int nreal, int per_xcd) {
-  __shared__ ZTile2Lds<CW * NR> sm;
+  __shared__ ZTile2HLds<CW, NR> sm;
   const int logical = xcd_logical(blockIdx.x, per_xcd);
   if (logical >= nreal) return;  // whole block, before any barrier
   member_offset(a);
@@ -1186,7 +1210,7 @@ __global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))
 template <int CW, int NR>
 __global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_ens_zt2(const MemberDev* __restrict__ m, int n,
                                                          int pass, int total, int per_xcd) {
-  __shared__ ZTile2Lds<CW * NR> sm;
+  __shared__ ZTile2HLds<CW, NR> sm;
   const int item = xcd_logical(blockIdx.x, per_xcd);
   if (item >= total) return;  // whole block, before any barrier
   const MemberDev& md = m[member_of<1>(m, n, item)];
