@@ -5,7 +5,7 @@
 // the library is compiled with -ffp-contract=off, so that a face flux or a
 // cell update rounds exactly like the NumPy code it replaces. The only
 // expected differences from NumPy are last-bit differences of the powers
-// (pow_pos of pmw_pow.h is correctly rounded in all but ~4e-6 of cases,
+// (pow_pos of pmw_pow.h is correctly rounded in all but ~1e-6 of cases,
 // glibc's pow, which NumPy calls, in all but 8e-4) and of exp/cos in the
 // initial conditions.
 #pragma once
@@ -90,15 +90,23 @@ __device__ __forceinline__ void recon4(double s0, double s1, double s2,
 // pow_pos; POW_FIRST also keeps its unrounded value in `ref` (the first RK
 // stage of a fused direction); POW_NEXT continues from `ref` (the later
 // stages, whose face values differ from the first stage's by a relative
-// ~1e-4 or less: pow_pos_near, with pow_pos for larger changes).
-enum PowMode { POW_PLAIN = 0, POW_FIRST = 1, POW_NEXT = 2 };
+// ~1e-4 or less: pow_pos_near, with pow_pos for larger changes); POW_TRY is
+// POW_NEXT without the pow_pos: *ok is set false where the continuation
+// could round differently from pow_pos (about 4e-5 of faces), and the
+// caller recomputes such a face with POW_PLAIN after its other faces, so
+// that the common path has no pow_pos inside it.
+enum PowMode { POW_PLAIN = 0, POW_FIRST = 1, POW_NEXT = 2, POW_TRY = 3 };
 
-__device__ __forceinline__ double face_power(double q, int mode, PowRef* ref) {
+__device__ __forceinline__ double face_power(double q, int mode, PowRef* ref, bool* ok) {
   if (mode == POW_PLAIN) return pow_pos(q, kGamma);
   double r;
+  if (mode == POW_TRY) {
+    *ok = pow_pos_near_try(q, kGamma, *ref, r);
+    return r;
+  }
   if (mode == POW_NEXT && pow_pos_near_try(q, kGamma, *ref, r)) return r;
-  // first stage, or a continuation too close to a rounding boundary (about
-  // one argument in 1400): pow_pos; only the first stage keeps the reference
+  // first stage, or a continuation too close to a rounding boundary:
+  // pow_pos; only the first stage keeps the reference
   PowRef scratch;
   return pow_pos_ref(q, kGamma, mode == POW_FIRST ? *ref : scratch);
 }
@@ -108,13 +116,13 @@ __device__ __forceinline__ void flux_x_face(const double val[4],
                                             const double d3[4], double hyr,
                                             double hyt, double hv,
                                             double F[4], int pmode = POW_PLAIN,
-                                            PowRef* ref = nullptr) {
+                                            PowRef* ref = nullptr, bool* ok = nullptr) {
   const double rho = val[0] + hyr;
   const double ir = 1.0 / rho;
   const double u = div_rn(val[1], rho, ir);
   const double w = div_rn(val[2], rho, ir);
   const double t = div_rn(val[3] + hyt, rho, ir);
-  const double p = kC0 * face_power(rho * t, pmode, ref);
+  const double p = kC0 * face_power(rho * t, pmode, ref, ok);
   const double ru = rho * u;
   F[0] = ru - hv * d3[0];
   F[1] = (rho * (u * u) + p) - hv * d3[1];
@@ -128,13 +136,13 @@ __device__ __forceinline__ void flux_z_face(const double val[4], double d3[4],
                                             double hyr, double hyt,
                                             double hyp, double hv, bool wall,
                                             double G[4], int pmode = POW_PLAIN,
-                                            PowRef* ref = nullptr) {
+                                            PowRef* ref = nullptr, bool* ok = nullptr) {
   const double rho = val[0] + hyr;
   const double ir = 1.0 / rho;
   const double u = div_rn(val[1], rho, ir);
   double w = div_rn(val[2], rho, ir);
   const double t = div_rn(val[3] + hyt, rho, ir);
-  const double p = kC0 * face_power(rho * t, pmode, ref) - hyp;
+  const double p = kC0 * face_power(rho * t, pmode, ref, ok) - hyp;
   if (wall) {
     w = 0.0;
     d3[0] = 0.0;

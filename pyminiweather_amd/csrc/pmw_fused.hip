@@ -126,12 +126,22 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
   // ---- stage 2: faces 4..NT-4, cells 4..NT-5, s1 -> s1 (in place)
   {
     const int j = t;
+    bool ok = true;
     if (j >= 4 && j <= NT - 4) {
       double v4[4], d3[4], F[4];
 #pragma unroll
       for (int v = 0; v < kNv; ++v)
         recon4(s1[v][j - 2], s1[v][j - 1], s1[v][j], s1[v][j + 1], v4[v], d3[v]);
-      flux_x_face(v4, d3, hyr, hyt, a.hv, F, POW_NEXT, &pref);
+      flux_x_face(v4, d3, hyr, hyt, a.hv, F, POW_TRY, &pref, &ok);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) fl[v][j] = F[v];
+    }
+    if (!ok) {  // the face again with pow_pos (rare)
+      double v4[4], d3[4], F[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v)
+        recon4(s1[v][j - 2], s1[v][j - 1], s1[v][j], s1[v][j + 1], v4[v], d3[v]);
+      flux_x_face(v4, d3, hyr, hyt, a.hv, F);
 #pragma unroll
       for (int v = 0; v < kNv; ++v) fl[v][j] = F[v];
     }
@@ -161,12 +171,22 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
   // ---- stage 3: faces 6..NT-6, cells 6..NT-7 -> out
   {
     const int j = t;
+    bool ok = true;
     if (j >= 6 && j <= NT - 6) {
       double v4[4], d3[4], F[4];
 #pragma unroll
       for (int v = 0; v < kNv; ++v)
         recon4(s1[v][j - 2], s1[v][j - 1], s1[v][j], s1[v][j + 1], v4[v], d3[v]);
-      flux_x_face(v4, d3, hyr, hyt, a.hv, F, POW_NEXT, &pref);
+      flux_x_face(v4, d3, hyr, hyt, a.hv, F, POW_TRY, &pref, &ok);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) fl[v][j] = F[v];
+    }
+    if (!ok) {  // the face again with pow_pos (rare)
+      double v4[4], d3[4], F[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v)
+        recon4(s1[v][j - 2], s1[v][j - 1], s1[v][j], s1[v][j + 1], v4[v], d3[v]);
+      flux_x_face(v4, d3, hyr, hyt, a.hv, F);
 #pragma unroll
       for (int v = 0; v < kNv; ++v) fl[v][j] = F[v];
     }
@@ -259,12 +279,22 @@ __device__ __forceinline__ void dir_xd_tile(const DirArgs& a, const Grid& g, int
   for (int stage = 0; stage < 3; ++stage) {
     const int m = 2 + 2 * stage;  // faces m..NT-m, cells m..NT-1-m
     auto& src_s = stage == 0 ? s0 : s1;
+    bool ok = true;
     if (t >= m && t <= NT - m) {
       double v4[4], d3[4], F[4];
 #pragma unroll
       for (int v = 0; v < kNv; ++v)
         recon4(src_s[v][t - 2], src_s[v][t - 1], src_s[v][t], src_s[v][t + 1], v4[v], d3[v]);
-      flux_x_face(v4, d3, hyr, hyt, a.hv, F, stage == 0 ? POW_FIRST : POW_NEXT, &pref);
+      flux_x_face(v4, d3, hyr, hyt, a.hv, F, stage == 0 ? POW_FIRST : POW_TRY, &pref, &ok);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) fl[v][t] = F[v];
+    }
+    if (!ok) {  // the face again with pow_pos (rare)
+      double v4[4], d3[4], F[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v)
+        recon4(src_s[v][t - 2], src_s[v][t - 1], src_s[v][t], src_s[v][t + 1], v4[v], d3[v]);
+      flux_x_face(v4, d3, hyr, hyt, a.hv, F);
 #pragma unroll
       for (int v = 0; v < kNv; ++v) fl[v][t] = F[v];
     }
@@ -827,6 +857,7 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
 #pragma unroll
   for (int stage = 0; stage < 3; ++stage) {
     const int e = 2 + 2 * stage;
+    bool ok[2] = {true, true};
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       const int j = jj[c], k = kk[c], u = sl[c];
@@ -836,7 +867,20 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
         for (int v = 0; v < kNv; ++v)
           recon4(s[v][u - 2 * CW], s[v][u - CW], s[v][u], s[v][u + CW], v4[v], d3[v]);
         flux_z_face(v4, d3, hz[0][j], hz[1][j], hz[2][j], a.hv, k == 0 || k == nz,
-                    G, stage == 0 ? POW_FIRST : POW_NEXT, &pref[c]);
+                    G, stage == 0 ? POW_FIRST : POW_TRY, &pref[c], &ok[c]);
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) fl[v][u] = G[v];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (!ok[c]) {  // the face again with pow_pos (rare)
+        const int j = jj[c], k = kk[c], u = sl[c];
+        double v4[4], d3[4], G[4];
+#pragma unroll
+        for (int v = 0; v < kNv; ++v)
+          recon4(s[v][u - 2 * CW], s[v][u - CW], s[v][u], s[v][u + CW], v4[v], d3[v]);
+        flux_z_face(v4, d3, hz[0][j], hz[1][j], hz[2][j], a.hv, k == 0 || k == nz, G);
 #pragma unroll
         for (int v = 0; v < kNv; ++v) fl[v][u] = G[v];
       }
@@ -916,12 +960,22 @@ inline int dense_z_blocks(int nx, int nz, int cw, int nr) {
 
 template <int CW, int NR>
 __device__ __forceinline__ void dir_zt2d_tile(const DirArgs& a, const Grid& g, int b,
-                                              ZTile2Lds<CW * NR>& sm, bool corners) {
+                                              ZTile2HLds<CW, NR>& sm, bool corners) {
   constexpr int NC = CW * NR, NT = NC / 2, C = NR - 12;
   auto& s = sm.s;
   auto& fl = sm.fl;
+  auto& hz = sm.hz;
   const int nx = g.nx, nz = g.nz, t = threadIdx.x;
   const int Ez = nz + 4;
+  static_assert(NT >= NR, "one thread per tile row fills the face hydrostatics");
+  if (t < NR) {  // a tile row is one row k of one column group
+    const int gr = b * C + t - 4;
+    const int k = gr >= 0 ? gr % Ez - 2 : -3;
+    const bool f = k >= 0 && k <= nz;
+    hz[0][t] = f ? a.hyr_int[k] : 0.0;
+    hz[1][t] = f ? a.hyt_int[k] : 0.0;
+    hz[2][t] = f ? a.hyp_int[k] : 0.0;
+  }
   const int col = t % CW;
   const gdouble* hy = a.hy_cell;
   const int plo = kHs, phi = nz + kHs - 1;
@@ -991,6 +1045,7 @@ __device__ __forceinline__ void dir_zt2d_tile(const DirArgs& a, const Grid& g, i
 #pragma unroll
   for (int stage = 0; stage < 3; ++stage) {
     const int e = 2 + 2 * stage;
+    bool ok[2] = {true, true};
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       const int j = jj[c], k = kk[c], u = sl[c];
@@ -999,8 +1054,21 @@ __device__ __forceinline__ void dir_zt2d_tile(const DirArgs& a, const Grid& g, i
 #pragma unroll
         for (int v = 0; v < kNv; ++v)
           recon4(s[v][u - 2 * CW], s[v][u - CW], s[v][u], s[v][u + CW], v4[v], d3[v]);
-        flux_z_face(v4, d3, a.hyr_int[k], a.hyt_int[k], a.hyp_int[k], a.hv, k == 0 || k == nz,
-                    G, stage == 0 ? POW_FIRST : POW_NEXT, &pref[c]);
+        flux_z_face(v4, d3, hz[0][j], hz[1][j], hz[2][j], a.hv, k == 0 || k == nz,
+                    G, stage == 0 ? POW_FIRST : POW_TRY, &pref[c], &ok[c]);
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) fl[v][u] = G[v];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (!ok[c]) {  // the face again with pow_pos (rare)
+        const int j = jj[c], k = kk[c], u = sl[c];
+        double v4[4], d3[4], G[4];
+#pragma unroll
+        for (int v = 0; v < kNv; ++v)
+          recon4(s[v][u - 2 * CW], s[v][u - CW], s[v][u], s[v][u + CW], v4[v], d3[v]);
+        flux_z_face(v4, d3, hz[0][j], hz[1][j], hz[2][j], a.hv, k == 0 || k == nz, G);
 #pragma unroll
         for (int v = 0; v < kNv; ++v) fl[v][u] = G[v];
       }
@@ -1056,7 +1124,7 @@ __device__ __forceinline__ void dir_zt2d_tile(const DirArgs& a, const Grid& g, i
 template <int CW, int NR>
 __global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_dir_zt2d(DirArgs a, Grid g, int nreal,
                                                           int per_xcd) {
-  __shared__ ZTile2Lds<CW * NR> sm;
+  __shared__ ZTile2HLds<CW, NR> sm;
   const int logical = xcd_logical(blockIdx.x, per_xcd);
   if (logical >= nreal) return;  // whole block, before any barrier
   member_offset(a);
@@ -1197,7 +1265,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 template <int CW, int NR>
 __global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_ens_zt2d(const MemberDev* __restrict__ m, int n,
                                                           int pass, int total, int per_xcd) {
-  __shared__ ZTile2Lds<CW * NR> sm;
+  __shared__ ZTile2HLds<CW, NR> sm;
   const int item = xcd_logical(blockIdx.x, per_xcd);
   if (item >= total) return;  // whole block, before any barrier
   const MemberDev& md = m[member_of<1>(m, n, item)];
