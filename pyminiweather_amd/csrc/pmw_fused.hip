@@ -620,6 +620,14 @@ __global__ __launch_bounds__(192) void k_dir_z(DirArgs a, Grid g, int ncolblk, i
 }
 
 // ------------------------------------------------------------- z tiles --
+// LDS of a one-cell-per-thread z tile: the x tile's arrays and the tile
+// rows' face hydrostatics (rho, rho*theta, p at the lower face of each tile
+// row), read from HBM once per tile; all CW columns of a row read one word.
+template <int NT, int NR>
+struct ZTileLds : XTileLds<NT> {
+  double hz[3][NR];
+};
+
 // z direction as LDS tiles, the x kernel's scheme turned on its side: a block
 // of NT threads holds CW columns x NR = NT/CW rows (slot t = row*CW + col,
 // tile row j = logical row r0-6+j), runs the three stages in LDS (stage 1 on
@@ -627,13 +635,14 @@ __global__ __launch_bounds__(192) void k_dir_z(DirArgs a, Grid g, int ncolblk, i
 // and rebuilds the wall halo rows of every stage state from its edge row, as
 // set_bc_z does between the stages (ics/bcs.py:64-148). Same operations in
 // the same order as the marching kernel, so results are bitwise equal.
-template <int CW, int NT>
+template <int CW, int NT, bool CONT = true>
 __device__ __forceinline__ void dir_zt_tile(const DirArgs& a, const Grid& g, int c0, int r0,
-                                            XTileLds<NT>& sm, bool corners) {
+                                            ZTileLds<NT, NT / CW>& sm, bool corners) {
   constexpr int NR = NT / CW, H = 6, C = NR - 2 * H;
   auto& s0 = sm.s0;
   auto& s1 = sm.s1;
   auto& fl = sm.fl;
+  auto& hz = sm.hz;
   const int nx = g.nx, nz = g.nz, t = threadIdx.x;
   const int j = t / CW, col = t % CW;
   const int i = c0 + col;     // interior column
@@ -646,8 +655,14 @@ __device__ __forceinline__ void dir_zt_tile(const DirArgs& a, const Grid& g, int
   const bool own = colok && k >= r0 && k < r0 + C && k < nz;
   const bool own_lo = r0 == 0, own_hi = r0 <= nz - 1 && nz - 1 < r0 + C;
   gdouble* const tmp_own = a.tmp_interior ? a.tmp : nullptr;
-  // slot of the edge row a wall slot is built from
-  const int edge_slot = wlo ? t - k * CW : t - (k - (nz - 1)) * CW;
+  static_assert(NT >= NR, "one thread per tile row fills the face hydrostatics");
+  if (t < NR) {
+    const int kr = r0 - H + t;
+    const bool f = kr >= 0 && kr <= nz;
+    hz[0][t] = f ? a.hyr_int[kr] : 0.0;
+    hz[1][t] = f ? a.hyt_int[kr] : 0.0;
+    hz[2][t] = f ? a.hyp_int[kr] : 0.0;
+  }
   {
     double q[4] = {0.0, 0.0, 0.0, 0.0};
     if (colok) {
@@ -684,20 +699,30 @@ __device__ __forceinline__ void dir_zt_tile(const DirArgs& a, const Grid& g, int
   }
   const double id = 1.0 / a.d;
   const double src = (a.src_w && colok && k >= 0 && k < nz) ? a.src_w[(long long)k * nx + i] : 0.0;
+  PowRef pref;  // CONT: this face's stage-1 power, continued in stages 2 and 3
   __syncthreads();
 #pragma unroll
   for (int stage = 0; stage < 3; ++stage) {
     double (*sin)[NT] = stage == 0 ? s0 : s1;
     const int e = 2 + 2 * stage;
     // faces: tile rows e .. NR-e (face k between logical rows k-1 and k)
+    bool ok = true;
     if (colok && j >= e && j <= NR - e && k >= 0 && k <= nz) {
       double v4[4], d3[4], G[4];
 #pragma unroll
       for (int v = 0; v < kNv; ++v)
         recon4(sin[v][t - 2 * CW], sin[v][t - CW], sin[v][t], sin[v][t + CW], v4[v], d3[v]);
-      // the plain power: this tile's 72 VGPRs leave no room for the
-      // stage-1 reference at 6 waves per SIMD (results are the same)
-      flux_z_face(v4, d3, a.hyr_int[k], a.hyt_int[k], a.hyp_int[k], a.hv, k == 0 || k == nz, G);
+      flux_z_face(v4, d3, hz[0][j], hz[1][j], hz[2][j], a.hv, k == 0 || k == nz, G,
+                  !CONT ? POW_PLAIN : stage == 0 ? POW_FIRST : POW_TRY, &pref, &ok);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) fl[v][t] = G[v];
+    }
+    if (CONT && !ok) {  // the face again with pow_pos (rare)
+      double v4[4], d3[4], G[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v)
+        recon4(sin[v][t - 2 * CW], sin[v][t - CW], sin[v][t], sin[v][t + CW], v4[v], d3[v]);
+      flux_z_face(v4, d3, hz[0][j], hz[1][j], hz[2][j], a.hv, k == 0 || k == nz, G);
 #pragma unroll
       for (int v = 0; v < kNv; ++v) fl[v][t] = G[v];
     }
@@ -716,6 +741,30 @@ __device__ __forceinline__ void dir_zt_tile(const DirArgs& a, const Grid& g, int
       if (stage < 2) {
 #pragma unroll
         for (int v = 0; v < kNv; ++v) s1[v][t] = q[v];
+        // The wall halo rows of the new stage state, written by the edge
+        // row's thread: no thread reads them in this phase (an update reads
+        // its own cell only), and a tile holding a wall has its edge row in
+        // the window, updated at every stage.
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+          if (k == (side == 0 ? 0 : nz - 1)) {
+            const int pe = side == 0 ? plo : phi;
+            const int step = side == 0 ? -CW : CW;
+#pragma unroll
+            for (int h = 1; h <= 2; ++h) {
+              const int kh = side == 0 ? -h : nz - 1 + h;  // the halo row's k
+              double w[4];
+#pragma unroll
+              for (int v = 0; v < kNv; ++v) w[v] = wall_halo(v, q[v], hy[pe], hy[kh + kHs]);
+#pragma unroll
+              for (int v = 0; v < kNv; ++v) s1[v][t + h * step] = w[v];
+              if (stage == 1 && (side == 0 ? own_lo : own_hi)) {  // set_bc_z of stage 3
+#pragma unroll
+                for (int v = 0; v < kNv; ++v) a.tmp[g.at(v, kh + kHs, pc)] = w[v];
+              }
+            }
+          }
+        }
       }
       if (own) {
         gdouble* dst = stage == 0 ? a.s1 : stage == 1 ? tmp_own : a.out;
@@ -727,20 +776,6 @@ __device__ __forceinline__ void dir_zt_tile(const DirArgs& a, const Grid& g, int
     }
     if (stage == 2) break;
     __syncthreads();
-    // wall halo rows of the new stage state from its edge row
-    if (colok && (wlo || whi)) {
-      const int pe = wlo ? plo : phi;
-      double q[4];
-#pragma unroll
-      for (int v = 0; v < kNv; ++v) q[v] = wall_halo(v, s1[v][edge_slot], hy[pe], hy[k + kHs]);
-#pragma unroll
-      for (int v = 0; v < kNv; ++v) s1[v][t] = q[v];
-      if (stage == 1 && ((wlo && own_lo) || (whi && own_hi))) {  // set_bc_z of stage 3
-#pragma unroll
-        for (int v = 0; v < kNv; ++v) a.tmp[g.at(v, k + kHs, pc)] = q[v];
-      }
-    }
-    __syncthreads();
   }
 }
 
@@ -750,7 +785,7 @@ template <int CW, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_dir_zt(
     DirArgs a, Grid g, int ncolgrp, int nreal, int per_xcd) {
   constexpr int C = NT / CW - 12;
-  __shared__ XTileLds<NT> sm;
+  __shared__ ZTileLds<NT, NT / CW> sm;
   const int logical = xcd_logical(blockIdx.x, per_xcd);
   if (logical >= nreal) return;  // whole block, before any barrier
   member_offset(a);
@@ -1251,7 +1286,7 @@ template <int CW, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_ens_zt(
     const MemberDev* __restrict__ m, int n, int pass, int total, int per_xcd) {
   constexpr int C = NT / CW - 12;
-  __shared__ XTileLds<NT> sm;
+  __shared__ ZTileLds<NT, NT / CW> sm;
   const int item = xcd_logical(blockIdx.x, per_xcd);
   if (item >= total) return;  // whole block, before any barrier
   const MemberDev& md = m[member_of<1>(m, n, item)];

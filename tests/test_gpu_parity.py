@@ -439,13 +439,16 @@ FUSED_TUNINGS = [(0, 256, 8), (0, 256, 16), (0, 512, 24), (0, 512, 32), (0, 512,
 @pytest.mark.parametrize("ic,nx,nz", [("collision", 60, 30), ("gravity", 40, 20),
                                       ("thermal", 12, 7), ("collision", 2, 2),
                                       ("density-current", 257, 37),
-                                      ("collision", 500, 70)])
+                                      ("collision", 500, 70), ("thermal", 40, 55),
+                                      ("collision", 24, 120)])
 @pytest.mark.parametrize("tuning", FUSED_TUNINGS)
 def test_fused_directions_bitwise_equal_per_stage(ic, nx, nz, tuning):
     """One launch per direction (all three RK stages, temporal blocking) and
     one launch per stage leave bit-identical state and state_tmp, halos and
     corners included, for awkward sizes (tiles and segments not dividing the
-    grid, grids smaller than one tile or segment)."""
+    grid, grids smaller than one tile or segment; nz = 55 and 120 put the
+    top edge row and its wall rows in the halo of a z tile below the one
+    that owns them, for 52- and 116-row tiles)."""
     p = O.make_params(nx, nz, ic, cfl=1.5, max_speed=450.0)
     fa, _ = gpu_fields(p)
     fb, _ = gpu_fields(p)
