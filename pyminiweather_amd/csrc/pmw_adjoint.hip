@@ -648,6 +648,7 @@ int pmw_tape_forward(double* tape, long long tape_len, int members,
       a.tmp = gptr(base + 2 * slot);
       a.out = gptr(base + 3 * slot);
       a.d = dir == 0 ? dx : dz;
+      a.id = 1.0 / a.d;
       a.hv = dir == 0 ? hv_coeff_x : hv_coeff_z;
       const int rc = dir == 0 ? launch_dir_x(a, g, x_nt, st, members)
                               : launch_dir_z(a, g, z_seg, st, members);

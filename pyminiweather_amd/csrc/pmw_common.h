@@ -33,6 +33,27 @@ __host__ __device__ inline PMW_GLOBAL T* gptr(T* p) {
   return (PMW_GLOBAL T*)p;
 }
 
+// A state element addressed from a block-uniform base (the start of a row
+// or tile rows of one variable plane, kept in scalar registers) by a 32-bit
+// byte offset per lane: one global access with a scalar base, instead of a
+// 64-bit address computed per lane and access.
+// The base goes through readfirstlane so that the compiler keeps it in
+// scalar registers instead of folding the variable's plane offset into a
+// per-lane 64-bit address.
+__device__ __forceinline__ unsigned long long sgpr64(unsigned long long u) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ double ld_off(const gdouble* base, unsigned boff) {
+  const unsigned long long b = sgpr64((unsigned long long)base);
+  return *(const gdouble*)((const PMW_GLOBAL char*)b + boff);
+}
+__device__ __forceinline__ void st_off(gdouble* base, unsigned boff, double v) {
+  const unsigned long long b = sgpr64((unsigned long long)base);
+  *(gdouble*)((PMW_GLOBAL char*)b + boff) = v;
+}
+
 // pyminiweather/data/constants.py:13-27 (rd is 287 in a float Enum -> 287.0)
 constexpr double kHvBeta = 0.05;
 constexpr double kP0 = 1.0e5;

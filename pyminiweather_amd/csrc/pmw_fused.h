@@ -18,6 +18,8 @@ struct DirArgs {
   const gdouble* hyp_int;   // hy_pressure_int (z faces)
   const gdouble* src_w;     // (nz, nx) w source or nullptr
   double d;                // dx or dz
+  double id;               // RN(1/d), set with d (a host division, so no
+                           // kernel spends an IEEE division on it)
   double dt1, dt2, dt3;    // dt/3, dt/2, dt
   double hv;               // -hv_beta * d / (16 * dt_full)
   gdouble* s1 = nullptr;   // optional: receives the stage-1 interior

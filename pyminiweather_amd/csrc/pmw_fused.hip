@@ -49,7 +49,10 @@ __device__ __forceinline__ int xcd_logical(int b, int per_xcd) {
   return (b % kNumXcd) * per_xcd + b / kNumXcd;
 }
 
+// Periodic column of cell c; the division only for the few lanes outside
+// the row (the tile halos at the row ends).
 __device__ __forceinline__ int wrap_cell(int c, int nx) {
+  if (c >= 0 && c < nx) return c;
   int w = c % nx;
   return w < 0 ? w + nx : w;
 }
@@ -78,20 +81,23 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
   const int cell = i0 - H + t;
   const int cw = wrap_cell(cell, nx);
   const long long srow = (long long)k * nx;
+  const long long rowoff = (long long)prow * g.ld;         // block-uniform
+  const unsigned bw = (unsigned)(cw + kHs) * 8u;            // byte offsets in the row
+  const unsigned bc = (unsigned)(cell + kHs) * 8u;          // (used where cell >= -2)
   {
     double q[4];
 #pragma unroll
-    for (int v = 0; v < kNv; ++v) q[v] = a.in[g.at(v, prow, cw + kHs)];
+    for (int v = 0; v < kNv; ++v) q[v] = ld_off(a.in + v * g.plane + rowoff, bw);
 #pragma unroll
     for (int v = 0; v < kNv; ++v) s0[v][t] = q[v];
     if (cell == -2 || cell == -1 || cell == nx || cell == nx + 1) {
 #pragma unroll
-      for (int v = 0; v < kNv; ++v) a.st[g.at(v, prow, cell + kHs)] = q[v];
+      for (int v = 0; v < kNv; ++v) st_off(a.st + v * g.plane + rowoff, bc, q[v]);
     }
   }
   const double hyr = a.hy_cell[prow], hyt = a.hyt_cell[prow];
   const double src = a.src_w ? a.src_w[srow + cw] : 0.0;
-  const double id = 1.0 / a.d;
+  const double id = a.id;
   __syncthreads();
   // face j is computed by thread j in all three stages, which keeps its
   // stage-1 pressure power for the later stages (POW_FIRST / POW_NEXT)
@@ -119,7 +125,7 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
     for (int v = 0; v < kNv; ++v) s1[v][t] = s0[v][t] + a.dt1 * td[v];
     if (a.s1 && cell >= i0 && cell < i0 + C && cell < nx) {
 #pragma unroll
-      for (int v = 0; v < kNv; ++v) a.s1[g.at(v, prow, cell + kHs)] = s1[v][t];
+      for (int v = 0; v < kNv; ++v) st_off(a.s1 + v * g.plane + rowoff, bc, s1[v][t]);
     }
   }
   __syncthreads();
@@ -164,7 +170,7 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
     const bool halo = cell == -2 || cell == -1 || cell == nx || cell == nx + 1;
     if (own || halo) {
 #pragma unroll
-      for (int v = 0; v < kNv; ++v) a.tmp[g.at(v, prow, cell + kHs)] = q[v];
+      for (int v = 0; v < kNv; ++v) st_off(a.tmp + v * g.plane + rowoff, bc, q[v]);
     }
   }
   __syncthreads();
@@ -198,7 +204,8 @@ __device__ __forceinline__ void dir_x_tile(const DirArgs& a, const Grid& g, int 
     for (int v = 0; v < kNv; ++v) td[v] = tend_of(fl[v][t], fl[v][t + 1], a.d, id);
     if (a.src_w) td[2] = td[2] + src;
 #pragma unroll
-    for (int v = 0; v < kNv; ++v) a.out[g.at(v, prow, cell + kHs)] = s0[v][t] + a.dt3 * td[v];
+    for (int v = 0; v < kNv; ++v)
+      st_off(a.out + v * g.plane + rowoff, bc, s0[v][t] + a.dt3 * td[v]);
   }
 }
 
@@ -272,7 +279,7 @@ __device__ __forceinline__ void dir_xd_tile(const DirArgs& a, const Grid& g, int
   // hydrostatic values; faces across a row boundary feed no correct cell
   const double hyr = a.hy_cell[prow], hyt = a.hyt_cell[prow];
   const double src = (a.src_w && row_ok) ? a.src_w[(long long)r * nx + cw] : 0.0;
-  const double id = 1.0 / a.d;
+  const double id = a.id;
   PowRef pref;  // this face's stage-1 pressure power, continued in stages 2 and 3
   __syncthreads();
 #pragma unroll
@@ -424,7 +431,7 @@ __device__ __forceinline__ bool z_stage_step(double w[4][4], double F[4], const 
     flux_z_face(v4, d3, a.hyr_int[f], a.hyt_int[f], a.hyp_int[f], a.hv, f == 0 || f == nz, G);
     const int c = f - 1;  // window row 1
     if (c >= lo) {
-      const double id = 1.0 / a.d;
+      const double id = a.id;
       double td[4];
 #pragma unroll
       for (int v = 0; v < kNv; ++v) td[v] = tend_of(F[v], G[v], a.d, id);
@@ -697,7 +704,7 @@ __device__ __forceinline__ void dir_zt_tile(const DirArgs& a, const Grid& g, int
       buf[g.at(v, nz + 3, cc)] = wall_halo(v, f, hy[phi], hy[nz + 3]);
     }
   }
-  const double id = 1.0 / a.d;
+  const double id = a.id;
   const double src = (a.src_w && colok && k >= 0 && k < nz) ? a.src_w[(long long)k * nx + i] : 0.0;
   PowRef pref;  // CONT: this face's stage-1 power, continued in stages 2 and 3
   __syncthreads();
@@ -886,7 +893,7 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
       buf[g.at(v, nz + 3, cc)] = wall_halo(v, f, hy[phi], hy[nz + 3]);
     }
   }
-  const double id = 1.0 / a.d;
+  const double id = a.id;
   PowRef pref[2];  // each thread computes the faces at its two slots in every stage
   __syncthreads();
 #pragma unroll
@@ -1074,7 +1081,7 @@ __device__ __forceinline__ void dir_zt2d_tile(const DirArgs& a, const Grid& g, i
     }
   }
   gdouble* const tmp_own = a.tmp_interior ? a.tmp : nullptr;
-  const double id = 1.0 / a.d;
+  const double id = a.id;
   PowRef pref[2];  // each thread computes the faces at its two slots in every stage
   __syncthreads();
 #pragma unroll
@@ -1191,6 +1198,7 @@ struct MemberDev {
   const double* src;   // w source or nullptr
   int nx, nz;
   double dx, dz, dt1, dt2, dt3, hvx, hvz;
+  double idx, idz;       // RN(1/dx), RN(1/dz)
   int x_start, z_start;  // first work item of this member in the x / z lists
   int x_ntiles, z_ncol;  // x tiles per row (dense x tiles: blocks of the member), z column groups
 };
@@ -1210,6 +1218,7 @@ __device__ __forceinline__ DirArgs member_args(const MemberDev& m, int dir, int 
   a.hyp_int = gptr(m.hypi);
   a.src_w = m.src ? gptr(m.src) : nullptr;
   a.d = dir == 0 ? m.dx : m.dz;
+  a.id = dir == 0 ? m.idx : m.idz;
   a.dt1 = m.dt1;
   a.dt2 = m.dt2;
   a.dt3 = m.dt3;
@@ -1490,6 +1499,8 @@ int plan_members(const pmw_member* mem, int n, int x_nt, int z_seg, MemberDev* d
       d.nz = m.nz;
       d.dx = m.dx;
       d.dz = m.dz;
+      d.idx = 1.0 / m.dx;
+      d.idz = 1.0 / m.dz;
       d.dt1 = m.dt / 3;
       d.dt2 = m.dt / 2;
       d.dt3 = m.dt / 1;

@@ -699,6 +699,7 @@ int evolve_impl(double* state, double* state_tmp, double* state_work,
         a.hyp_int = gptr(hy_pressure_int);
         a.src_w = gptr(src_w);
         a.d = dir == 0 ? dx : dz;
+        a.id = 1.0 / a.d;
         a.dt1 = dts[0];
         a.dt2 = dts[1];
         a.dt3 = dts[2];
