@@ -2,6 +2,8 @@
 // reference (quadmath powq, rounded once to double) and glibc's pow.
 // Usage: pow_check N lo hi y  -> prints "N mismatch_cr mismatch_glibc
 // glibc_vs_cr special_mismatches max_rel_err_of_pow_pos_fast"
+//        pow_check N lo hi y q EMAX [RXERR] -> pow_pos_near == pow_pos count
+//        (reference reciprocal scaled by 1 +- RXERR); prints "N diff inrange"
 //        pow_check N lo hi y near EMAX -> pow_pos_near continued from a
 // reference at x0 to x0 (1 + e), |e| log-uniform in [1e-15, EMAX] with a
 // random sign (one argument in 16 equal to x0); prints "N near_vs_cr
@@ -52,6 +54,9 @@ int main(int argc, char** argv) {
   }
   if (argc >= 7 && argv[5][0] == 'q') {  // pow_pos_near == pow_pos, no quadmath
     const double emax = atof(argv[6]);
+    // optional: the reference's reciprocal scaled by (1 + rxerr), as an
+    // approximate reciprocal would leave it
+    const double rxerr = argc >= 8 ? atof(argv[7]) : 0.0;
     long long diff = 0, inrange = 0;
     for (long long i = 0; i < n; ++i) {
       const double u = (double)(next(&s) >> 11) * 0x1p-53;
@@ -62,6 +67,7 @@ int main(int argc, char** argv) {
       const double x = x0 * (1.0 + e);
       pmw::PowRef ref;
       (void)pmw::pow_pos_ref(x0, y, ref);
+      ref.rx *= 1.0 + ((i & 1) ? rxerr : -rxerr);
       diff += pmw::pow_pos_near(x, y, ref) != pmw::pow_pos(x, y);
       inrange += fabs(x - x0) <= 0x1p-10 * x0;
     }

@@ -73,6 +73,18 @@ def test_pow_near_equals_pow_pos(checker, lo, hi):
     assert int(out[1]) == 0, out
 
 
+@pytest.mark.parametrize("rxerr", ["1e-15", "1e-14"])
+def test_pow_near_insensitive_to_reference_reciprocal(checker, rxerr):
+    """On the device the reference's 1/x0 is the hardware reciprocal with
+    two Newton steps (an ulp or two), not an IEEE division. With 1/x0 off
+    by 1e-15 or 1e-14 relative (~9 and ~90 ulps) pow_pos_near still returns
+    pow_pos bit for bit (at 1e-13 it does not: the e^2 terms are evaluated
+    from eh, which carries the reciprocal's error)."""
+    out = subprocess.run([checker, "2000000", "50", "2000", GAMMA, "q", "9.7e-4", rxerr],
+                         capture_output=True, text=True).stdout.split()
+    assert int(out[1]) == 0, out
+
+
 def test_tables_regenerate_identically():
     """pmw_pow_tables.h is what gen_pow_tables.py writes."""
     r = subprocess.run(["python3", os.path.join(CSRC, "gen_pow_tables.py")],
