@@ -322,10 +322,13 @@ struct AdjDirArgs {
 // before wrote (faces 2+2m..NR-2-2m read slots 1+2m..NR-2-2m, inside the
 // cells 3+2(m-1)..NR-4-2(m-1) of stage m-1), and slots outside the domain
 // keep the zero they were loaded with.
+// The face cotangents of the third differences are -hv lamF, so only those
+// of the reconstructed values are kept; a cell forms the others from la
+// (96 B of LDS per slot).
 template <int NT>
 struct AdjDirLds {
   double st[kNv][NT];  // forcing state of the stage being reversed
-  double fw[8][NT];    // face cotangents of (vals, d3)
+  double fw[kNv][NT];  // face cotangents of the reconstructed values
   double la[kNv][NT];  // lam3, then lamS2, then lamS1
 };
 
@@ -336,7 +339,8 @@ __device__ __forceinline__ int adj_xcd_logical(int b, int per_xcd) {
 }
 
 template <int DIR, int CW, int NT>
-__global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline, int nreal,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_adj_dir(
+    AdjDirArgs a, Grid g, int nline, int nreal,
                                                 int per_xcd) {
   constexpr int NR = NT / CW, H = 7, C = NR - 2 * H, S = DIR == 0 ? 1 : CW;
   __shared__ AdjDirLds<NT> sm;
@@ -415,7 +419,7 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
     const bool zout = DIR == 1 && (p < 0 || p > nz);
     if (zout) {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) sm.fw[q][t] = 0.0;
+      for (int q = 0; q < kNv; ++q) sm.fw[q][t] = 0.0;
     } else if (j >= 2 + 2 * m && j <= NR - 2 - 2 * m && (DIR == 0 || col + (blk % nline) * CW < nx)) {
       double val[4], d3[4], lf[4], lv[4], ld[4];
 #pragma unroll
@@ -428,10 +432,7 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
       else
         adj_flux_z(val, hy_r, hy_t, a.hv, p == 0 || p == nz, lf, lv, ld);
 #pragma unroll
-      for (int v = 0; v < kNv; ++v) {
-        sm.fw[v][t] = lv[v];
-        sm.fw[4 + v][t] = ld[v];
-      }
+      for (int v = 0; v < kNv; ++v) sm.fw[v][t] = lv[v];
     }
     __syncthreads();
     // gravity needs lin[w] at this slot: read before lin may be overwritten
@@ -440,11 +441,25 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
     double l[4] = {0.0, 0.0, 0.0, 0.0};
     const bool cell = inside && j >= 3 + 2 * m && j <= NR - 4 - 2 * m;
     if (cell) {
+      // d3 cotangent of the face at slot sl (the face between slots sl-S
+      // and sl, at position fp): -hv lamF, lamF formed as the face phase
+      // forms it; zero outside 0..nz and for rho' at the walls (z)
+      auto ldf = [&](int sl, int fp, int v) {
+        if (DIR == 1 && (fp < 0 || fp > nz || (v == 0 && (fp == 0 || fp == nz)))) return 0.0;
+        return -a.hv * (sd * (lin[v][sl] - lin[v][sl - S]));
+      };
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int sl = t + (2 - q) * S;
+      for (int v = 0; v < kNv; ++v) {
+        double lam[5];  // lin at slots t-2S .. t+2S
 #pragma unroll
-        for (int v = 0; v < kNv; ++v) l[v] += kW4[q] * sm.fw[v][sl] + kD3[q] * sm.fw[4 + v][sl];
+        for (int i = 0; i < 5; ++i) lam[i] = lin[v][t + (i - 2) * S];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {  // face at slot t+(2-q)S, position p+2-q
+          const int fp = p + 2 - q;
+          const bool zero = DIR == 1 && (fp < 0 || fp > nz || (v == 0 && (fp == 0 || fp == nz)));
+          const double ld = zero ? 0.0 : -a.hv * (sd * (lam[4 - q] - lam[3 - q]));
+          l[v] += kW4[q] * sm.fw[v][t + (2 - q) * S] + kD3[q] * ld;
+        }
       }
       if (DIR == 1) {
         if (p == 0 || p == nz - 1) {
@@ -458,7 +473,7 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
               if (f < 0 || f > nz) continue;
               const int sl = t + (f - p) * S;
 #pragma unroll
-              for (int v = 0; v < kNv; ++v) h[v] += kW4[q] * sm.fw[v][sl] + kD3[q] * sm.fw[4 + v][sl];
+              for (int v = 0; v < kNv; ++v) h[v] += kW4[q] * sm.fw[v][sl] + kD3[q] * ldf(sl, f, v);
             }
             l[0] += h[0];
             l[1] += h[1] * a.hy_cell[r + kHs] / a.hy_cell[prow];
@@ -472,6 +487,7 @@ __global__ __launch_bounds__(NT) void k_adj_dir(AdjDirArgs a, Grid g, int nline,
       // next forcing state; st is free once the faces are done
 #pragma unroll
       for (int v = 0; v < kNv; ++v) sm.st[v][t] = nq[v];
+      __syncthreads();  // every cell has read its neighbours' la
       if (cell) {
 #pragma unroll
         for (int v = 0; v < kNv; ++v) {
