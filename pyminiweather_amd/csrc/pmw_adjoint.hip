@@ -50,25 +50,49 @@ __device__ __forceinline__ int wrapc(int c, int n) {
   return w < 0 ? w + n : w;
 }
 
-// Primitive variables of a face and dp/dq (q = rho*theta, p = C0 q^gamma):
-// one division and one pow per face.
+// q^(gamma-1) for dp/dq = gamma C0 q^(gamma-1) (q = rho*theta at a face): a
+// derivative, so a few-ulp power.
+__device__ __forceinline__ double dpow(double q) { return pow_pos_fast(q, kGamma - 1.0); }
+
+// dpow(q) continued from dpow(q0) = p0 at a nearby q0: q^a = p0 (1 + e)^a,
+// e = (q - q0)/q0, by the binomial series to e^4 (truncation below 3e-17
+// for |e| <= 2^-10); ok is set false for larger changes (and NaN), where
+// the caller evaluates dpow(q) itself. 1/q0 is the hardware reciprocal
+// (2^29 ulp) with one Newton step, ~2^-46 relative, which moves the result
+// by less than 0.4 |e| 2^-46 < 4e-17. The adjoint tiles' later stages
+// reverse the same face.
+__device__ __forceinline__ double dpow_near(double q, double q0, double p0, bool& ok) {
+  double r = __builtin_amdgcn_rcp(q0);
+  r = fma(fma(-q0, r, 1.0), r, r);
+  const double e = (q - q0) * r;
+  ok = fabs(e) <= 0x1p-10;
+  double c = -0x1.54c5337464d59p-5;          // a(a-1)(a-2)(a-3)/24, a = gamma - 1
+  c = c * e + 0x1.0628bd48c9ba4p-4;          // a(a-1)(a-2)/6
+  c = c * e + -0x1.eba2540426299p-4;         // a(a-1)/2
+  c = c * e + 0x1.99e2b8f2ca162p-2;          // a
+  return p0 + p0 * (e * c);
+}
+
+// Primitive variables of a face and dp/dq (q = rho*theta, p = C0 q^gamma)
+// from pw = dpow(q): one division per face.
 __device__ __forceinline__ void face_prims(const double val[4], double hyr, double hyt,
-                                           double& u, double& w, double& t, double& dp) {
+                                           double pw, double& u, double& w, double& t,
+                                           double& dp) {
   const double rho = val[0] + hyr;
   const double ir = 1.0 / rho;
   const double q = val[3] + hyt;
   u = val[1] * ir;
   w = val[2] * ir;
   t = q * ir;
-  dp = (kGamma * kC0) * pow_pos_fast(q, kGamma - 1.0);  // a derivative: few-ulp power
+  dp = (kGamma * kC0) * pw;
 }
 
 // Transposed x-flux Jacobian: cotangents of (vals, d3) from lamF.
 __device__ __forceinline__ void adj_flux_x(const double val[4], double hyr, double hyt,
                                            double hv, const double lf[4], double lv[4],
-                                           double ld[4]) {
+                                           double ld[4], double pw) {
   double u, w, t, dp;
-  face_prims(val, hyr, hyt, u, w, t, dp);
+  face_prims(val, hyr, hyt, pw, u, w, t, dp);
   lv[0] = -(u * u) * lf[1] - (u * w) * lf[2] - (u * t) * lf[3];
   lv[1] = lf[0] + 2.0 * u * lf[1] + w * lf[2] + t * lf[3];
   lv[2] = u * lf[2];
@@ -80,9 +104,9 @@ __device__ __forceinline__ void adj_flux_x(const double val[4], double hyr, doub
 // Transposed z-flux Jacobian; at the wall faces w and d3[rho'] are cut.
 __device__ __forceinline__ void adj_flux_z(const double val[4], double hyr, double hyt,
                                            double hv, bool wall, const double lf[4],
-                                           double lv[4], double ld[4]) {
+                                           double lv[4], double ld[4], double pw) {
   double u, w, t, dp;
-  face_prims(val, hyr, hyt, u, w, t, dp);
+  face_prims(val, hyr, hyt, pw, u, w, t, dp);
   if (wall) {
     lv[0] = lv[1] = lv[2] = 0.0;
     lv[3] = dp * lf[2];
@@ -170,7 +194,8 @@ __global__ __launch_bounds__(kAdjXThreads) void k_adj_x(AdjArgs a, Grid g, int n
 #pragma unroll
     for (int v = 0; v < kNv; ++v)
       lf[v] = a.sd * (a.lam[g.at(v, prow, col[2])] - a.lam[g.at(v, prow, col[1])]);
-    adj_flux_x(val, a.hy_cell[prow], a.hyt_cell[prow], a.hv, lf, lv, ld);
+    adj_flux_x(val, a.hy_cell[prow], a.hyt_cell[prow], a.hv, lf, lv, ld,
+               dpow(val[3] + a.hyt_cell[prow]));
 #pragma unroll
     for (int v = 0; v < kNv; ++v) {
       fw[v][t] = lv[v];
@@ -245,7 +270,7 @@ __global__ __launch_bounds__(64 * kAdjZWaves) void k_adj_z(AdjArgs a, Grid g, in
         lf[v] = a.sd * (above - below);
       }
       adj_flux_z(val, a.hyr_int[face], a.hyt_int[face], a.hv, face == 0 || face == nz, lf, lv,
-                 ld);
+                 ld, dpow(val[3] + a.hyt_int[face]));
 #pragma unroll
       for (int v = 0; v < kNv; ++v) {
         fw[v][s][lane] = lv[v];
@@ -371,11 +396,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
   }
   const bool own = inside && j >= H && j < H + C && (DIR == 0 ? p < nx : true);
   // lam3 (zero outside the domain), lamS2 cleared
-  double l3[4], l2[4], l1[4];
+  // acc: the init terms lam3 + lamS2 + lamS1, summed as the stages finish
+  double acc[4];
 #pragma unroll
   for (int v = 0; v < kNv; ++v) {
-    l3[v] = inside ? a.lam[off + g.at(v, prow, pcol)] : 0.0;
-    sm.la[v][t] = l3[v];
+    acc[v] = inside ? a.lam[off + g.at(v, prow, pcol)] : 0.0;
+    sm.la[v][t] = acc[v];
   }
   // this thread's slot of a forcing state (wall halo rows rebuilt)
   auto fetch_state = [&](const double* src, double q[4]) {
@@ -404,6 +430,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     hy_r = a.hyr_int[p];
     hy_t = a.hyt_int[p];
   }
+  double q0 = 1.0, p0 = 1.0;  // this face's stage-3 power reference
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < 3; ++m) {  // m = 0: stage 3 (forcing S2), 1: stage 2, 2: stage 1
@@ -417,6 +444,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     if (m < 2) fetch_state(a.s[stage - 1], nq);
     // faces at slots 2+2m .. NR-2-2m (z: faces outside 0..nz are zero)
     const bool zout = DIR == 1 && (p < 0 || p > nz);
+    bool pok = true;
     if (zout) {
 #pragma unroll
       for (int q = 0; q < kNv; ++q) sm.fw[q][t] = 0.0;
@@ -427,10 +455,35 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
         recon4(sm.st[v][t - 2 * S], sm.st[v][t - S], sm.st[v][t], sm.st[v][t + S], val[v], d3[v]);
 #pragma unroll
       for (int v = 0; v < kNv; ++v) lf[v] = sd * (lin[v][t] - lin[v][t - S]);
+      // the face's power: evaluated at stage 3, continued at stages 2 and 1
+      const double q = val[3] + hy_t;
+      double pw;
+      if (m == 0) {
+        pw = dpow(q);
+        q0 = q;
+        p0 = pw;
+      } else {
+        pw = dpow_near(q, q0, p0, pok);
+      }
       if (DIR == 0)
-        adj_flux_x(val, hy_r, hy_t, a.hv, lf, lv, ld);
+        adj_flux_x(val, hy_r, hy_t, a.hv, lf, lv, ld, pw);
       else
-        adj_flux_z(val, hy_r, hy_t, a.hv, p == 0 || p == nz, lf, lv, ld);
+        adj_flux_z(val, hy_r, hy_t, a.hv, p == 0 || p == nz, lf, lv, ld, pw);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) sm.fw[v][t] = lv[v];
+    }
+    if (!pok) {  // the face again with the power itself (rare)
+      double val[4], d3[4], lf[4], lv[4], ld[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v)
+        recon4(sm.st[v][t - 2 * S], sm.st[v][t - S], sm.st[v][t], sm.st[v][t + S], val[v], d3[v]);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) lf[v] = sd * (lin[v][t] - lin[v][t - S]);
+      const double pw = dpow(val[3] + hy_t);
+      if (DIR == 0)
+        adj_flux_x(val, hy_r, hy_t, a.hv, lf, lv, ld, pw);
+      else
+        adj_flux_z(val, hy_r, hy_t, a.hv, p == 0 || p == nz, lf, lv, ld, pw);
 #pragma unroll
       for (int v = 0; v < kNv; ++v) sm.fw[v][t] = lv[v];
     }
@@ -448,17 +501,31 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
         if (DIR == 1 && (fp < 0 || fp > nz || (v == 0 && (fp == 0 || fp == nz)))) return 0.0;
         return -a.hv * (sd * (lin[v][sl] - lin[v][sl - S]));
       };
+      // Away from the walls (all of x) the four faces' d3 cotangents sum
+      // to -hv sd times the fourth difference of lambda at the cell
+      // (kD3 = -1, 3, -3, 1 over differences of neighbours).
+      const bool interior = DIR == 0 || (p >= 2 && p <= nz - 3);
+      const double c4 = -a.hv * sd;
 #pragma unroll
       for (int v = 0; v < kNv; ++v) {
         double lam[5];  // lin at slots t-2S .. t+2S
 #pragma unroll
         for (int i = 0; i < 5; ++i) lam[i] = lin[v][t + (i - 2) * S];
+        // transposed 4-tap reconstruction (kW4 = a, b, b, a)
+        double w = kW4a * (sm.fw[v][t + 2 * S] + sm.fw[v][t - S]) +
+                   kW4b * (sm.fw[v][t + S] + sm.fw[v][t]);
+        if (interior) {
+          // sum_q kD3[q] (lam[4-q] - lam[3-q]) = 4 (l1 + l3) - (l0 + l4) - 6 l2
+          const double d4 = (4.0 * (lam[1] + lam[3]) - (lam[0] + lam[4])) - 6.0 * lam[2];
+          l[v] = w + c4 * d4;
+        } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {  // face at slot t+(2-q)S, position p+2-q
-          const int fp = p + 2 - q;
-          const bool zero = DIR == 1 && (fp < 0 || fp > nz || (v == 0 && (fp == 0 || fp == nz)));
-          const double ld = zero ? 0.0 : -a.hv * (sd * (lam[4 - q] - lam[3 - q]));
-          l[v] += kW4[q] * sm.fw[v][t + (2 - q) * S] + kD3[q] * ld;
+          for (int q = 0; q < 4; ++q) {  // face at slot t+(2-q)S, position p+2-q
+            const int fp = p + 2 - q;
+            const bool zero = fp < 0 || fp > nz || (v == 0 && (fp == 0 || fp == nz));
+            if (!zero) w += kD3[q] * (c4 * (lam[4 - q] - lam[3 - q]));
+          }
+          l[v] = w;
         }
       }
       if (DIR == 1) {
@@ -492,18 +559,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 #pragma unroll
         for (int v = 0; v < kNv; ++v) {
           sm.la[v][t] = l[v];  // in place: this stage's faces are done
-          if (m == 0) l2[v] = l[v]; else l1[v] = l[v];
+          acc[v] += l[v];
         }
       }
       __syncthreads();
     } else if (own) {
 #pragma unroll
       for (int v = 0; v < kNv; ++v) {
-        double r = l[v];
-        r += l3[v];
-        r += l2[v];
-        r += l1[v];
-        a.out[off + g.at(v, prow, pcol)] = r;
+        a.out[off + g.at(v, prow, pcol)] = l[v] + acc[v];
       }
     }
   }
