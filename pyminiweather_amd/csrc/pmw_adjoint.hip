@@ -79,7 +79,7 @@ __device__ __forceinline__ void face_prims(const double val[4], double hyr, doub
                                            double pw, double& u, double& w, double& t,
                                            double& dp) {
   const double rho = val[0] + hyr;
-  const double ir = 1.0 / rho;
+  const double ir = rcp_approx(rho);  // a derivative needs no IEEE quotient
   const double q = val[3] + hyt;
   u = val[1] * ir;
   w = val[2] * ir;
