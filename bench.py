@@ -239,25 +239,35 @@ def bench_adjoint(nsteps=400, fd_cells=8):
     (target: forward run from the thermal IC + N(0, 0.1 K) theta' noise,
     numpy default_rng(1)); gradient w.r.t. the initial theta' with sqrt(N)
     checkpoints. Times the checkpointed forward sweep and the backward sweep
-    (segment recompute into the tape + adjoint kernels), and checks
+    (segment recompute into the tape + adjoint kernels) twice: with every
+    segment recomputed ("recompute_all") and with the forward sweep's tapes
+    kept where they fit in device memory (the solver's default; the headline
+    figures). "adjoint_cost_vs_plain_forward" is the kept run's forward +
+    backward less the plain forward, over the plain forward. Checks
     `fd_cells` random cells against central differences."""
     import numpy as np
     import torch
     from pyminiweather_amd.adjoint import AdjointSolver
     from pyminiweather_amd.autotune import tune_tape
     p, f = _thermal_400x200()
-    solver = AdjointSolver(p, f, nsteps)
-    tune_tape(solver)
-    w = solver.row_weight()[:, None]
-    theta0 = (f.state[3, 2:-2, 2:-2] / w).clone()
-    noise = torch.from_numpy(np.random.default_rng(1).normal(0.0, 0.1, (200, 400))).cuda()
-    target = solver.final_rhot(theta0 + noise)
-    solver.loss_and_grad(theta0, target)  # warm-up
-    s0 = solver.state_from_theta(theta0)
-    (final, ckpts), t_fwd = _timed(lambda: solver.forward(s0))
-    lam = torch.zeros_like(final)
-    lam[:, 3, 2:-2, 2:-2] = final[:, 3, 2:-2, 2:-2] - target
-    lam0, t_bwd = _timed(lambda: solver.backward(lam, ckpts))
+    w = None
+    timings = {}
+    for keep in (0, "auto"):  # every segment recomputed; the forward's tapes kept
+        solver = AdjointSolver(p, f, nsteps, keep_tapes=keep)
+        tune_tape(solver)
+        if w is None:
+            w = solver.row_weight()[:, None]
+            theta0 = (f.state[3, 2:-2, 2:-2] / w).clone()
+            noise = torch.from_numpy(np.random.default_rng(1).normal(0.0, 0.1, (200, 400)))
+            target = solver.final_rhot(theta0 + noise.cuda())
+        solver.loss_and_grad(theta0, target)  # warm-up
+        s0 = solver.state_from_theta(theta0)
+        (final, ckpts), t_fwd = _timed(lambda: solver.forward(s0))
+        lam = torch.zeros_like(final)
+        lam[:, 3, 2:-2, 2:-2] = final[:, 3, 2:-2, 2:-2] - target
+        lam0, t_bwd = _timed(lambda: solver.backward(lam, ckpts))
+        timings[keep] = (t_fwd, t_bwd)
+    t_fwd0, t_bwd0 = timings[0]
     grad = lam0[0, 3, 2:-2, 2:-2] * w
     cells = 400 * 200 * nsteps
     fd = {}
@@ -282,6 +292,11 @@ def bench_adjoint(nsteps=400, fd_cells=8):
             "roofline_frac": round(cells / (t_fwd + t_bwd) * FWD_ADJ_BYTES_PER_CELL_STEP
                                    / 1e9 / HBM_PEAK_GBS, 4),
             "fd_rel_l2": fd, "fd_cells": fd_cells,
+            "kept_segments": solver.kept_segments,
+            "segments": -(-nsteps // solver.every),
+            "adjoint_cost_vs_plain_forward": (t_fwd + t_bwd - t_fwd0) / t_fwd0,
+            "recompute_all": {"forward_ms": t_fwd0 * 1e3, "backward_ms": t_bwd0 * 1e3,
+                              "adjoint_cost_vs_forward": t_bwd0 / t_fwd0},
             "tuning": list(solver.tuning)}
 
 
@@ -290,24 +305,40 @@ def bench_batched_adjoint(members=64, nsteps=400, iters=20):
     forward run from the thermal IC + N(0, 0.1 K) theta' noise (numpy
     default_rng(2 + m)); `iters` steepest-descent iterations on the initial
     theta', forward and checkpointed adjoint batched in one launch per
-    direction for all members."""
+    direction for all members. Timed as in bench_adjoint: with every segment
+    recomputed and with the forward sweep's tapes kept where they fit."""
     import numpy as np
     import torch
     from pyminiweather_amd.adjoint import AdjointSolver
     from pyminiweather_amd.autotune import tune_tape
     p, f = _thermal_400x200()
-    solver = AdjointSolver(p, f, nsteps, members=members)
-    tune_tape(solver)
-    w = solver.row_weight()[:, None]
+    # first with every segment recomputed in the backward sweep (sqrt(N)
+    # memory), then with the forward sweep's tapes kept where they fit
+    # (the solver's default); the forward-only time of the former is the
+    # plain forward the adjoint's cost is also quoted against
+    plain = AdjointSolver(p, f, nsteps, members=members, keep_tapes=0)
+    tune_tape(plain)
+    w = plain.row_weight()[:, None]
     theta0 = (f.state[3, 2:-2, 2:-2] / w).expand(members, 200, 400).clone()
     noise = np.stack([np.random.default_rng(2 + m).normal(0.0, 0.1, (200, 400))
                       for m in range(members)])
-    target = solver.final_rhot(theta0 + torch.from_numpy(noise).cuda())
-    s0 = solver.state_from_theta(theta0)
-    (final, ckpts), t_fwd = _timed(lambda: solver.forward(s0))
-    lam = torch.zeros_like(final)
-    lam[:, 3, 2:-2, 2:-2] = final[:, 3, 2:-2, 2:-2] - target
-    _, t_bwd = _timed(lambda: solver.backward(lam, ckpts))
+    target = plain.final_rhot(theta0 + torch.from_numpy(noise).cuda())
+    s0 = plain.state_from_theta(theta0)
+
+    def sweep(solver):
+        (final, ckpts), t_f = _timed(lambda: solver.forward(s0))
+        lam = torch.zeros_like(final)
+        lam[:, 3, 2:-2, 2:-2] = final[:, 3, 2:-2, 2:-2] - target
+        _, t_b = _timed(lambda: solver.backward(lam, ckpts))
+        return t_f, t_b
+
+    t_fwd0, t_bwd0 = sweep(plain)
+    tuning = plain.tuning
+    del plain
+    torch.cuda.empty_cache()
+    solver = AdjointSolver(p, f, nsteps, members=members, tuning=tuning)
+    solver.final_rhot(theta0)  # warm-up
+    t_fwd, t_bwd = sweep(solver)
     cells = members * 400 * 200 * nsteps
     out = {"workload": f"{members} members 400x200, {nsteps} RK3 steps, per-member "
                        "seeded targets (seeds 2..), steepest descent on theta'",
@@ -315,6 +346,10 @@ def bench_batched_adjoint(members=64, nsteps=400, iters=20):
            "unit": "fwd+adjoint member-cell-steps/s",
            "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
            "adjoint_cost_vs_forward": t_bwd / t_fwd,
+           "kept_segments": solver.kept_segments, "segments": -(-nsteps // solver.every),
+           "adjoint_cost_vs_plain_forward": (t_fwd + t_bwd - t_fwd0) / t_fwd0,
+           "recompute_all": {"forward_ms": t_fwd0 * 1e3, "backward_ms": t_bwd0 * 1e3,
+                             "adjoint_cost_vs_forward": t_bwd0 / t_fwd0},
            "roofline_frac": round(cells / (t_fwd + t_bwd) * FWD_ADJ_BYTES_PER_CELL_STEP
                                    / 1e9 / HBM_PEAK_GBS, 4),
            "launch": "one launch per direction for all members, forward and adjoint "

@@ -13,7 +13,12 @@ recompute of segment s-1 runs on a second stream while the adjoint of
 segment s runs on the caller's stream (events order each adjoint after its
 recompute, and each recompute after the adjoint that last read its tape).
 Memory is O(sqrt(N)) states plus two segments' tapes (6 states per step);
-the recompute is one extra forward run.
+the recompute is one extra forward run. With ``keep_tapes`` (by default as
+many as fit in half of the device memory free when the solver is made) the
+forward sweep writes the full tapes of its last segments as it goes, and
+the backward sweep uses them without recomputing: HBM capacity traded for
+the recompute (the forward sweep pays the stage-state stores on those
+segments).
 
 ``EvolveFunction`` wraps the same as a ``torch.autograd.Function`` (initial
 interior state -> final interior state), so it composes with torch code.
@@ -121,7 +126,7 @@ def discrete_step_adjoint(params, fields, state_forcing, lam_out, dt, direction)
 
 class AdjointSolver:
     def __init__(self, params, fields, nsteps, checkpoint_every=None, tuning=None, members=1,
-                 adjoint_tuning=None):
+                 adjoint_tuning=None, keep_tapes="auto"):
         if params["ic_type"] == "injection":
             raise ValueError("the adjoint supports the periodic-x ICs only")
         self.p = dict(params)
@@ -149,9 +154,21 @@ class AdjointSolver:
         dev = fields.device
         n = min(self.every, self.nsteps)
         nseg = -(-self.nsteps // self.every)
-        self._tapes = [torch.zeros((6 * n + 1,) + self.bshape, dtype=torch.float64, device=dev)
+        tshape = (6 * n + 1,) + self.bshape
+        self._tapes = [torch.zeros(tshape, dtype=torch.float64, device=dev)
                        for _ in range(min(nseg, 2))]
         self._tape = self._tapes[0]
+        # tapes kept from the forward sweep for the last segments
+        tape_bytes = 8 * math.prod(tshape)
+        if keep_tapes == "auto":
+            free, _ = torch.cuda.mem_get_info(dev)
+            keep = int((free // 2) // tape_bytes)
+        else:
+            keep = int(keep_tapes)
+        keep = max(0, min(keep, nseg))
+        self._kept = {seg: torch.zeros(tshape, dtype=torch.float64, device=dev)
+                      for seg in range(nseg - keep, nseg)}
+        self._kept_owner = None  # the checkpoint list of the run the kept tapes hold
         # recompute stream (default priority), used when there are >= 2 segments
         self._side = torch.cuda.Stream(device=dev) if nseg > 1 else None
         self._ws = torch.zeros((1,) + self.bshape, dtype=torch.float64, device=dev)
@@ -189,22 +206,35 @@ class AdjointSolver:
         ckpts = []
         done = 0
         while done < self.nsteps:
+            seg = len(ckpts)
             ckpts.append(s.clone())
             n = min(self.every, self.nsteps - done)
-            s = self._tape_run(s, done, n, store=False)
+            if seg in self._kept:
+                s = self._tape_run(s, done, n, store=True, tape=self._kept[seg])
+            else:
+                s = self._tape_run(s, done, n, store=False)
             done += n
+        self._kept_owner = ckpts
         return s.clone().reshape(state0.shape), ckpts
+
+    @property
+    def kept_segments(self):
+        """Number of segments whose tapes the forward sweep keeps."""
+        return len(self._kept)
+
+    def _tape_of(self, seg, kept):
+        return self._kept[seg] if seg in kept else self._tapes[seg % 2]
 
     def _recompute(self, ckpts, seg):
         start = seg * self.every
         n = min(self.every, self.nsteps - start)
         self._tape_run(ckpts[seg], start, n, store=True, tape=self._tapes[seg % 2])
 
-    def _adjoint(self, lam, seg):
+    def _adjoint(self, lam, seg, kept=()):
         f = self.f
         start = seg * self.every
         n = min(self.every, self.nsteps - start)
-        tape = self._tapes[seg % 2]
+        tape = self._tape_of(seg, kept)
         _lib.call("pmw_tape_adjoint", ptr(tape), tape.numel(), self.members,
                   ptr(lam), ptr(self._ws), self._ws.numel(), ptr(f.hy_dens_cell),
                   ptr(f.hy_dens_theta_cell), ptr(f.hy_dens_int),
@@ -218,9 +248,15 @@ class AdjointSolver:
         return, that stream follows every launch made here."""
         lam = self._batch(lam_final).clone()
         nseg = len(ckpts)
-        if nseg == 1:
-            self._recompute(ckpts, 0)
-            self._adjoint(lam, 0)
+        # the kept tapes hold the latest forward run: used only for its own
+        # checkpoints (another run's are recomputed in full)
+        kept = self._kept if ckpts is self._kept_owner else {}
+        nrec = nseg - len(kept)  # segments 0 .. nrec-1 are recomputed
+        if nseg == 1 or nrec <= 1:
+            for seg in reversed(range(nseg)):
+                if seg not in kept:
+                    self._recompute(ckpts, seg)
+                self._adjoint(lam, seg, kept)
             return lam.reshape(lam_final.shape)
         main = torch.cuda.current_stream(self.f.device)
         side = self._side
@@ -233,11 +269,16 @@ class AdjointSolver:
                 ready[seg] = torch.cuda.Event()
                 ready[seg].record(side)
 
-        recompute_on_side(nseg - 1)
-        recompute_on_side(nseg - 2)
+        # the two latest recomputed segments first; they overlap the sweep
+        # of the kept ones
+        recompute_on_side(nrec - 1)
+        recompute_on_side(nrec - 2)
         for seg in reversed(range(nseg)):
+            if seg in kept:
+                self._adjoint(lam, seg, kept)
+                continue
             main.wait_event(ready.pop(seg))
-            self._adjoint(lam, seg)
+            self._adjoint(lam, seg, kept)
             if seg >= 2:  # tape seg % 2 is free once this adjoint is done
                 done = torch.cuda.Event()
                 done.record(main)
