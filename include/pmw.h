@@ -179,7 +179,7 @@ int pmw_evolve(double* state, double* state_tmp, double* state_work,
                int* reverse_direction, int nsteps, void* stream);
 
 /* pmw_evolve with explicit kernel choice and optional per-launch timing.
- * `tuning` (HOST, 3 ints, NULL = defaults {0, 512, 8128}):
+ * `tuning` (HOST, 3 ints, NULL = defaults {0, 512, 8096}):
  *   [0] mode: 0 = one fused launch per direction (all three RK stages,
  *       periodic x only; the injection BC always uses mode 1), 1 = one
  *       launch per RK stage (the pmw_rk_stage kernels);
@@ -190,6 +190,8 @@ int pmw_evolve(double* state, double* state_tmp, double* state_work,
  *   [2] fused z kernel: 8, 16, 24, 32, 48 or 64 = rows per block of the
  *       marching kernel (one wave per RK stage); 8064 / 4128 = LDS tiles of
  *       8 columns x 64 rows / 4 columns x 128 rows (all stages in LDS);
+ *       8096 = LDS tiles of 8 columns x 96 rows, one cell per thread, the
+ *       RK init in registers and the stage state updated in place;
  *       8128 = LDS tiles of 8 columns x 128 rows, two cells per thread;
  *       18128 = the 8128 tiles over the column groups laid end to end
  *       (each column with its two wall halo rows per side).

@@ -819,10 +819,11 @@ struct ZTile2HLds : ZTile2Lds<CW * NR> {
   double hz[3][NR];
 };
 
-template <int CW, int NR>
+// CPT cells per thread: 2 (tile rows j and j + NR/2) or 1.
+template <int CW, int NR, int CPT = 2>
 __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, int c0, int r0,
                                              ZTile2HLds<CW, NR>& sm, bool corners) {
-  constexpr int NC = CW * NR, NT = NC / 2, H = 6, C = NR - 2 * H;
+  constexpr int NC = CW * NR, NT = NC / CPT, H = 6, C = NR - 2 * H;
   auto& s = sm.s;
   auto& fl = sm.fl;
   auto& hz = sm.hz;
@@ -843,11 +844,11 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
     hz[1][t] = f ? a.hyt_int[k] : 0.0;
     hz[2][t] = f ? a.hyp_int[k] : 0.0;
   }
-  int jj[2], kk[2], sl[2];
-  bool wl[2], wh[2], own[2];
-  double S[2][4], src[2];
+  int jj[CPT], kk[CPT], sl[CPT];
+  bool wl[CPT], wh[CPT], own[CPT];
+  double S[CPT][4], src[CPT];
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < CPT; ++c) {
     sl[c] = t + c * NT;
     jj[c] = sl[c] / CW;
     kk[c] = r0 - H + jj[c];
@@ -894,14 +895,16 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
     }
   }
   const double id = a.id;
-  PowRef pref[2];  // each thread computes the faces at its two slots in every stage
+  PowRef pref[CPT];  // each thread computes the faces at its slots in every stage
   __syncthreads();
 #pragma unroll
   for (int stage = 0; stage < 3; ++stage) {
     const int e = 2 + 2 * stage;
-    bool ok[2] = {true, true};
+    bool ok[CPT];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < CPT; ++c) ok[c] = true;
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
       const int j = jj[c], k = kk[c], u = sl[c];
       if (colok && j >= e && j <= NR - e && k >= 0 && k <= nz) {
         double v4[4], d3[4], G[4];
@@ -915,7 +918,7 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
       }
     }
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < CPT; ++c) {
       if (!ok[c]) {  // the face again with pow_pos (rare)
         const int j = jj[c], k = kk[c], u = sl[c];
         double v4[4], d3[4], G[4];
@@ -930,7 +933,7 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
     __syncthreads();
     const double dts = stage == 0 ? a.dt1 : stage == 1 ? a.dt2 : a.dt3;
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < CPT; ++c) {
       const int j = jj[c], k = kk[c], u = sl[c];
       if (colok && j >= e && j <= NR - e - 1 && k >= 0 && k < nz) {
         double td[4];
@@ -1184,6 +1187,19 @@ __global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))
                        logical == 0);
 }
 
+// The same tiles with one cell per thread (CW x NR threads): at 8 x 96, two
+// 12-wave blocks per CU (50 KB of LDS each) at 6 waves per SIMD.
+template <int CW, int NR>
+__global__ __launch_bounds__(CW * NR) __attribute__((amdgpu_waves_per_eu(6))) void k_dir_zt1(
+    DirArgs a, Grid g, int ncolgrp, int nreal, int per_xcd) {
+  __shared__ ZTile2HLds<CW, NR> sm;
+  const int logical = xcd_logical(blockIdx.x, per_xcd);
+  if (logical >= nreal) return;  // whole block, before any barrier
+  member_offset(a);
+  dir_zt2_tile<CW, NR, 1>(a, g, (logical % ncolgrp) * CW, (logical / ncolgrp) * (NR - 12), sm,
+                          logical == 0);
+}
+
 // ------------------------------------------------------------- ensemble --
 // Device descriptor of one member of an ensemble (uneven grids).
 struct MemberDev {
@@ -1320,6 +1336,20 @@ __global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))
 }
 
 template <int CW, int NR>
+__global__ __launch_bounds__(CW * NR) __attribute__((amdgpu_waves_per_eu(6))) void k_ens_zt1(
+    const MemberDev* __restrict__ m, int n, int pass, int total, int per_xcd) {
+  __shared__ ZTile2HLds<CW, NR> sm;
+  const int item = xcd_logical(blockIdx.x, per_xcd);
+  if (item >= total) return;  // whole block, before any barrier
+  const MemberDev& md = m[member_of<1>(m, n, item)];
+  const DirArgs a = member_args(md, 1, pass & 1, pass >> 1);
+  const int local = item - md.z_start;
+  if ((long long)(local / md.z_ncol) * (NR - 12) >= md.nz) return;  // as in k_ens_x
+  dir_zt2_tile<CW, NR, 1>(a, Grid(md.nx, md.nz), (local % md.z_ncol) * CW,
+                          (local / md.z_ncol) * (NR - 12), sm, local == 0);
+}
+
+template <int CW, int NR>
 __global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_ens_zt2(const MemberDev* __restrict__ m, int n,
                                                          int pass, int total, int per_xcd) {
   __shared__ ZTile2HLds<CW, NR> sm;
@@ -1341,6 +1371,18 @@ int launch_zt2(const DirArgs& a, Grid g, hipStream_t st, int members) {
   const int per = (int)((nreal + kNumXcd - 1) / kNumXcd);
   k_dir_zt2<CW, NR><<<dim3(per * kNumXcd, members), CW * NR / 2, 0, st>>>(a, g, ncolgrp,
                                                                           (int)nreal, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
+template <int CW, int NR>
+int launch_zt1(const DirArgs& a, Grid g, hipStream_t st, int members) {
+  constexpr int C = NR - 12;
+  const int ncolgrp = (g.nx + CW - 1) / CW;
+  const long long nreal = (long long)ncolgrp * ((g.nz + C - 1) / C);
+  const int per = (int)((nreal + kNumXcd - 1) / kNumXcd);
+  k_dir_zt1<CW, NR><<<dim3(per * kNumXcd, members), CW * NR, 0, st>>>(a, g, ncolgrp,
+                                                                      (int)nreal, per);
   PMW_CHECK_LAUNCH();
   return PMW_OK;
 }
@@ -1406,11 +1448,12 @@ bool fused_x_tile_ok(int nt) {
   return nt == 256 || nt == 448 || nt == 512 || nt == 1256 || nt == 1512;
 }
 // z: 8..64 = rows per marching segment; columns*1000 + rows = LDS tiles:
-// 8064, 4128 (one cell per thread), 8128 (two cells per thread); 18128 =
-// 8128 as dense tiles (column groups end to end)
+// 8064, 4128 (one cell per thread, RK init in LDS), 8096 (one cell per
+// thread, RK init in registers, stage state updated in place), 8128 (two
+// cells per thread); 18128 = 8128 as dense tiles (column groups end to end)
 bool fused_z_seg_ok(int seg) {
   return seg == 8 || seg == 16 || seg == 24 || seg == 32 || seg == 48 || seg == 64 ||
-         seg == 8064 || seg == 4128 || seg == 8128 || seg == 18128;
+         seg == 8064 || seg == 4128 || seg == 8128 || seg == 18128 || seg == 8096;
 }
 
 int launch_dir_x(const DirArgs& a, Grid g, int nt, hipStream_t st, int members) {
@@ -1438,6 +1481,7 @@ int launch_dir_z(const DirArgs& a, Grid g, int seg, hipStream_t st, int members)
     case 8064: return launch_zt<8, 512>(a, g, st, members);
     case 4128: return launch_zt<4, 512>(a, g, st, members);
     case 8128: return launch_zt2<8, 128>(a, g, st, members);
+    case 8096: return launch_zt1<8, 96>(a, g, st, members);
     case 18128:
       if (dense_z_rows(g.nx, g.nz, 8) > (1LL << 30)) return PMW_EINVAL;
       return launch_zt2d<8, 128>(a, g, st, members);
@@ -1599,6 +1643,14 @@ int launch_ens_zt2d(const MemberDev* d, int n, int pass, int total, hipStream_t 
 }
 
 template <int CW, int NR>
+int launch_ens_zt1(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
+  const int per = ens_per_xcd(total);
+  k_ens_zt1<CW, NR><<<per * kNumXcd, CW * NR, 0, st>>>(d, n, pass, total, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
+template <int CW, int NR>
 int launch_ens_zt2(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
   const int per = ens_per_xcd(total);
   k_ens_zt2<CW, NR><<<per * kNumXcd, CW * NR / 2, 0, st>>>(d, n, pass, total, per);
@@ -1632,6 +1684,7 @@ int launch_ens(int dir, int x_nt, int z_seg, const MemberDev* d, int n, int pass
     case 8064: return launch_ens_zt<8, 512>(d, n, pass, plan.z_total, st);
     case 4128: return launch_ens_zt<4, 512>(d, n, pass, plan.z_total, st);
     case 8128: return launch_ens_zt2<8, 128>(d, n, pass, plan.z_total, st);
+    case 8096: return launch_ens_zt1<8, 96>(d, n, pass, plan.z_total, st);
     case 18128: return launch_ens_zt2d<8, 128>(d, n, pass, plan.z_total, st);
     default: return launch_ens_z<64>(d, n, pass, plan.z_total, st);
   }

@@ -631,9 +631,9 @@ namespace {
 struct Tuning {
   int mode;   // 0: one fused launch per direction; 1: one launch per RK stage
   int x_nt;   // fused x tiles: 256, 448, 512 threads per row tile; 1256, 1512 dense
-  int z_seg;  // fused z: rows per marching segment (8..64) or LDS tile code (8064, 4128, 8128)
+  int z_seg;  // fused z: rows per marching segment (8..64) or LDS tile code (8064, 4128, 8096, 8128)
 };
-constexpr Tuning kDefaultTuning = {0, 512, 8128};
+constexpr Tuning kDefaultTuning = {0, 512, 8096};
 
 int tuning_from(const int* t, Tuning& out) {
   out = kDefaultTuning;

@@ -194,7 +194,7 @@ def discrete_step(params, fields, mesh, state_init, state_forcing, state_out, dt
 # Kernel choice for evolve: (mode, fused x block size, fused z segment rows).
 # mode 0 = one fused launch per direction, 1 = one launch per RK stage. The
 # per-grid entries are filled by pyminiweather_amd.autotune.
-DEFAULT_TUNING = (0, 512, 8128)
+DEFAULT_TUNING = (0, 512, 8096)
 _tuning_table = {}
 
 

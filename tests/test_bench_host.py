@@ -12,6 +12,7 @@ import bench  # noqa: E402
 def test_fused_kernel_names_follow_the_tuning_codes():
     assert bench.fused_kernel_names((0, 512, 8128)) == ("k_dir_x<512>", "k_dir_zt2<8, 128>")
     assert bench.fused_kernel_names((0, 1256, 18128)) == ("k_dir_xd<256>", "k_dir_zt2d<8, 128>")
+    assert bench.fused_kernel_names((0, 512, 8096)) == ("k_dir_x<512>", "k_dir_zt1<8, 96>")
     assert bench.fused_kernel_names((0, 256, 4128)) == ("k_dir_x<256>", "k_dir_zt<4, 512>")
     assert bench.fused_kernel_names((0, 448, 8064)) == ("k_dir_x<448>", "k_dir_zt<8, 512>")
     assert bench.fused_kernel_names((0, 512, 16)) == ("k_dir_x<512>", "k_dir_z<16>")
@@ -24,7 +25,7 @@ def test_traffic_of_the_default_kernels_is_in_the_committed_summary():
     cell, at 2000 x 1000."""
     assert os.path.exists(bench.PROFILE_SUMMARY)
     cells = 2000 * 1000
-    for k in bench.fused_kernel_names((0, 512, 8128)):
+    for k in bench.fused_kernel_names((0, 512, 8096)):
         t = bench.traffic_from_profile(k)
         assert t is not None, k
         assert 60 * cells <= t <= bench.BYTES_PER_CELL_LAUNCH * cells, (k, t)
@@ -35,7 +36,7 @@ def test_valu_counts_of_the_default_kernels_are_in_the_committed_summary():
     """Each default kernel has its counted VALU instructions per launch, and
     the FP64 issue floor they imply lies below the kernel's traced average
     duration (a floor above it would mean the counter or the unit is wrong)."""
-    for k in bench.fused_kernel_names((0, 512, 8128)):
+    for k in bench.fused_kernel_names((0, 512, 8096)):
         n = bench.valu_from_profile(k)
         assert n is not None and n > 1e6, k
         floor_us = n / bench.VALU_ISSUE_PEAK * 1e6

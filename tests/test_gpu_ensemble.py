@@ -30,7 +30,7 @@ def _host(t):
 
 @pytest.mark.parametrize("tuning", [(0, 256, 16), (0, 512, 8), (0, 256, 32), (0, 512, 8064),
                                     (0, 512, 8128), (0, 256, 4128), (0, 1256, 8128),
-                                    (0, 1512, 24), (0, 1256, 18128)])
+                                    (0, 1512, 24), (0, 1256, 18128), (0, 1256, 8096)])
 def test_members_bitwise_equal_single_grid_solves(tuning):
     import pyminiweather_amd.solve as S
     from pyminiweather_amd.data import initialize_fields

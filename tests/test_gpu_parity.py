@@ -432,7 +432,7 @@ def _evolve_ex(p, f, nsteps, tuning, flag=0):
 
 
 FUSED_TUNINGS = [(0, 256, 8), (0, 256, 16), (0, 512, 24), (0, 512, 32), (0, 512, 8064),
-                 (0, 256, 8128), (0, 448, 4128), (0, 1256, 8128), (0, 1512, 48),
+                 (0, 256, 8128), (0, 448, 4128), (0, 1256, 8128), (0, 1512, 48), (0, 512, 8096),
                  (0, 512, 18128), (0, 1256, 18128)]
 
 
