@@ -57,6 +57,41 @@ __device__ __forceinline__ int wrap_cell(int c, int nx) {
   return w < 0 ? w + nx : w;
 }
 
+// Phase order between the waves of a block by flags in LDS instead of block
+// barriers, for tiles whose every phase reads and writes only a wave's own
+// cells and the 2 cells on either side of them along the tile's line (in
+// the one-cell z tiles a wave holds 8 whole rows of 8 columns): a wave waits
+// for its two neighbours alone, so one late wave no longer holds up the
+// whole block. wf[1 + w] = the last phase wave w finished; wf[0] and
+// wf[NW + 1] stay above every phase. Every wave passes every phase in order,
+// so the wave with the lowest phase never waits and the block cannot stall.
+// The flags are set before the block's first barrier, which publishes them.
+__device__ __forceinline__ void wave_flags_init(int* wf, int nw, int t) {
+  if (t < nw + 2)
+    __hip_atomic_store(&wf[t], (t == 0 || t == nw + 1) ? 1 << 20 : 0, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Phase P (> 0, increasing) of the calling wave is done; returns once both
+// neighbouring waves have done it too.
+__device__ __forceinline__ void wave_phase_done(int* wf, int P, int t) {
+  const int wv = t >> 6;
+  __builtin_amdgcn_sched_barrier(0);
+  // this wave's LDS writes of phase P are complete before its flag says so
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  if ((t & 63) == 0)
+    __hip_atomic_store(&wf[1 + wv], P, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  while (true) {
+    const int lo = __hip_atomic_load(&wf[wv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const int hi = __hip_atomic_load(&wf[2 + wv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (lo >= P && hi >= P) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  // the neighbours' LDS writes are read after their flags
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_sched_barrier(0);
+}
+
 // ------------------------------------------------------------------ x ------
 // LDS of one x tile: S on slots 0..NT-1 (cell = i0 - 6 + slot), the running
 // stage state (S1, then S2 in place) and the face fluxes (face j between
@@ -817,10 +852,14 @@ struct ZTile2Lds {
 template <int CW, int NR>
 struct ZTile2HLds : ZTile2Lds<CW * NR> {
   double hz[3][NR];
+  int wf[CW * NR / 64 + 2];  // wave phase flags (one-cell tiles)
 };
 
-// CPT cells per thread: 2 (tile rows j and j + NR/2) or 1.
-template <int CW, int NR, int CPT = 2>
+// CPT cells per thread: 2 (tile rows j and j + NR/2) or 1. WSYNC (one cell
+// per thread, 8 columns, so that a wave holds 8 whole tile rows): the phases
+// are ordered by wave flags instead of block barriers; every phase touches
+// only the wave's rows and 2 rows on either side.
+template <int CW, int NR, int CPT = 2, bool WSYNC = false>
 __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, int c0, int r0,
                                              ZTile2HLds<CW, NR>& sm, bool corners) {
   constexpr int NC = CW * NR, NT = NC / CPT, H = 6, C = NR - 2 * H;
@@ -837,6 +876,15 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
   const bool own_lo = r0 == 0, own_hi = r0 <= nz - 1 && nz - 1 < r0 + C;
   gdouble* const tmp_own = a.tmp_interior ? a.tmp : nullptr;
   static_assert(NT >= NR, "one thread per tile row fills the face hydrostatics");
+  static_assert(!WSYNC || (CPT == 1 && CW == 8 && NT % 64 == 0), "waves of 8 whole rows");
+  if (WSYNC) wave_flags_init(sm.wf, NT / 64, t);
+  // between phases: block barrier, or the neighbouring waves' flags
+  auto phase_done = [&](int P) {
+    if constexpr (WSYNC)
+      wave_phase_done(sm.wf, P, t);
+    else
+      __syncthreads();
+  };
   if (t < NR) {
     const int k = r0 - H + t;
     const bool f = k >= 0 && k <= nz;
@@ -930,7 +978,7 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
         for (int v = 0; v < kNv; ++v) fl[v][u] = G[v];
       }
     }
-    __syncthreads();
+    phase_done(2 * stage + 1);
     const double dts = stage == 0 ? a.dt1 : stage == 1 ? a.dt2 : a.dt3;
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
@@ -982,7 +1030,7 @@ __device__ __forceinline__ void dir_zt2_tile(const DirArgs& a, const Grid& g, in
       }
     }
     if (stage == 2) break;
-    __syncthreads();
+    phase_done(2 * stage + 2);
   }
 }
 
@@ -1188,7 +1236,8 @@ __global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))
 }
 
 // The same tiles with one cell per thread (CW x NR threads): at 8 x 96, two
-// 12-wave blocks per CU (50 KB of LDS each) at 6 waves per SIMD.
+// 12-wave blocks per CU (50 KB of LDS each) at 6 waves per SIMD, the waves
+// ordered by neighbour flags.
 template <int CW, int NR>
 __global__ __launch_bounds__(CW * NR) __attribute__((amdgpu_waves_per_eu(6))) void k_dir_zt1(
     DirArgs a, Grid g, int ncolgrp, int nreal, int per_xcd) {
@@ -1196,8 +1245,8 @@ __global__ __launch_bounds__(CW * NR) __attribute__((amdgpu_waves_per_eu(6))) vo
   const int logical = xcd_logical(blockIdx.x, per_xcd);
   if (logical >= nreal) return;  // whole block, before any barrier
   member_offset(a);
-  dir_zt2_tile<CW, NR, 1>(a, g, (logical % ncolgrp) * CW, (logical / ncolgrp) * (NR - 12), sm,
-                          logical == 0);
+  dir_zt2_tile<CW, NR, 1, true>(a, g, (logical % ncolgrp) * CW,
+                                (logical / ncolgrp) * (NR - 12), sm, logical == 0);
 }
 
 // ------------------------------------------------------------- ensemble --
@@ -1345,8 +1394,8 @@ __global__ __launch_bounds__(CW * NR) __attribute__((amdgpu_waves_per_eu(6))) vo
   const DirArgs a = member_args(md, 1, pass & 1, pass >> 1);
   const int local = item - md.z_start;
   if ((long long)(local / md.z_ncol) * (NR - 12) >= md.nz) return;  // as in k_ens_x
-  dir_zt2_tile<CW, NR, 1>(a, Grid(md.nx, md.nz), (local % md.z_ncol) * CW,
-                          (local / md.z_ncol) * (NR - 12), sm, local == 0);
+  dir_zt2_tile<CW, NR, 1, true>(a, Grid(md.nx, md.nz), (local % md.z_ncol) * CW,
+                                (local / md.z_ncol) * (NR - 12), sm, local == 0);
 }
 
 template <int CW, int NR>
