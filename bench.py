@@ -89,6 +89,8 @@ def fused_kernel_names(tuning):
     x = f"k_dir_xd<{xb - 1000}>" if xb > 1000 else f"k_dir_x<{xb}>"
     if zs == 18128:
         z = "k_dir_zt2d<8, 128>"
+    elif zs == 18096:
+        z = "k_dir_zt1d<8, 96>"
     elif zs == 8096:
         z = "k_dir_zt1<8, 96>"
     elif zs >= 1000:

@@ -193,8 +193,8 @@ int pmw_evolve(double* state, double* state_tmp, double* state_work,
  *       8096 = LDS tiles of 8 columns x 96 rows, one cell per thread, the
  *       RK init in registers and the stage state updated in place;
  *       8128 = LDS tiles of 8 columns x 128 rows, two cells per thread;
- *       18128 = the 8128 tiles over the column groups laid end to end
- *       (each column with its two wall halo rows per side).
+ *       18128 / 18096 = the 8128 / 8096 tiles over the column groups
+ *       laid end to end (each column with its two wall halo rows per side).
  * Both modes leave bit-identical `state` and `state_tmp`.
  * `kernel_ms` (HOST, 2 doubles, or NULL): if given, a HIP event is recorded
  * around every launch, the call blocks until they finish, and each launch's

@@ -22,9 +22,9 @@ from ._lib import ptr, stream_of
 X_BLOCKS = (256, 448, 512, 1256, 1512)
 # z: rows per marching segment, or an LDS tile code columns*1000 + rows
 # (8064, 4128: one cell per thread; 8096: one cell per thread with the RK
-# init in registers; 8128: two cells per thread; 18128: 8128 as dense
-# tiles, column groups end to end)
-Z_SEGMENTS = (8, 16, 24, 32, 48, 64, 8064, 4128, 8096, 8128, 18128)
+# init in registers; 8128: two cells per thread; 18128 / 18096: 8128 /
+# 8096 as dense tiles, column groups end to end)
+Z_SEGMENTS = (8, 16, 24, 32, 48, 64, 8064, 4128, 8096, 8128, 18128, 18096)
 
 
 def _time_grid(params, fields, tuning, steps, bufs):

@@ -1214,6 +1214,161 @@ __device__ __forceinline__ void dir_zt2d_tile(const DirArgs& a, const Grid& g, i
   }
 }
 
+// The dense tiles with one cell per thread (CW x NR threads, a wave holding 8
+// whole slot rows): the RK init in registers, the stage state updated in
+// place, the wall halo rows of each stage state written by the edge row's
+// thread in the update phase, and the waves ordered by neighbour flags. A
+// face that reads a wall halo row lies 2 slots inside the next stage's face
+// window, so its edge row was updated, and its halo rows written, in this
+// block at the stage before; halo rows of edge rows outside that window are
+// left unwritten, as no face of the block that is kept reads them.
+template <int CW, int NR>
+__device__ __forceinline__ void dir_zt1d_tile(const DirArgs& a, const Grid& g, int b,
+                                              ZTile2HLds<CW, NR>& sm, bool corners) {
+  constexpr int NT = CW * NR, C = NR - 12;
+  static_assert(CW == 8 && NT % 64 == 0, "waves of 8 whole slot rows");
+  auto& s = sm.s;
+  auto& fl = sm.fl;
+  auto& hz = sm.hz;
+  const int nx = g.nx, nz = g.nz, t = threadIdx.x;
+  const int Ez = nz + 4;
+  wave_flags_init(sm.wf, NT / 64, t);
+  if (t < NR) {  // a tile row is one row k of one column group
+    const int gr = b * C + t - 4;
+    const int k = gr >= 0 ? gr % Ez - 2 : -3;
+    const bool f = k >= 0 && k <= nz;
+    hz[0][t] = f ? a.hyr_int[k] : 0.0;
+    hz[1][t] = f ? a.hyt_int[k] : 0.0;
+    hz[2][t] = f ? a.hyp_int[k] : 0.0;
+  }
+  const int col = t % CW, u = t, j = t / CW;
+  const gdouble* hy = a.hy_cell;
+  const int plo = kHs, phi = nz + kHs - 1;
+  const int gr = b * C + j - 4;
+  const int grp = gr >= 0 ? gr / Ez : -1;
+  const int k = gr >= 0 ? gr - grp * Ez - 2 : -3;  // -3: padding before the first group
+  const int i = grp * CW + col;
+  const bool colok = gr >= 0 && i < nx;
+  const int pc = (colok ? i : 0) + kHs;
+  const bool wl = k == -2 || k == -1, wh = k == nz || k == nz + 1;
+  const bool own = colok && j >= 6 && j < NR - 6 && k >= 0 && k < nz;
+  double S[4];
+  {
+    double q[4] = {0.0, 0.0, 0.0, 0.0};
+    if (colok) {
+      if (k >= 0 && k < nz) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) q[v] = a.in[g.at(v, k + kHs, pc)];
+      } else if (wl || wh) {
+        const int pe = wl ? plo : phi;
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) q[v] = wall_halo(v, a.in[g.at(v, pe, pc)], hy[pe], hy[k + kHs]);
+        const int je = wl ? j - k : j - (k - (nz - 1));  // the edge row's tile row
+        if (je >= 6 && je < NR - 6) {  // set_bc_z of stage 1 into st
+#pragma unroll
+          for (int v = 0; v < kNv; ++v) a.st[g.at(v, k + kHs, pc)] = q[v];
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) {
+      s[v][u] = q[v];
+      S[v] = q[v];
+    }
+  }
+  const double src = (a.src_w && colok && k >= 0 && k < nz) ? a.src_w[(long long)k * nx + i] : 0.0;
+  if (corners && t < 8) {
+    // corner columns (padded 0, 1, nx+2, nx+3) of the halo rows of st, tmp
+    gdouble* buf = t < 4 ? a.st : a.tmp;
+    const int qq = t & 3;
+    const int cc = qq < 2 ? qq : nx + qq;
+#pragma unroll
+    for (int v = 0; v < kNv; ++v) {
+      const double e = buf[g.at(v, plo, cc)];
+      buf[g.at(v, 0, cc)] = wall_halo(v, e, hy[plo], hy[0]);
+      buf[g.at(v, 1, cc)] = wall_halo(v, e, hy[plo], hy[1]);
+      const double f = buf[g.at(v, phi, cc)];
+      buf[g.at(v, nz + 2, cc)] = wall_halo(v, f, hy[phi], hy[nz + 2]);
+      buf[g.at(v, nz + 3, cc)] = wall_halo(v, f, hy[phi], hy[nz + 3]);
+    }
+  }
+  gdouble* const tmp_own = a.tmp_interior ? a.tmp : nullptr;
+  const double id = a.id;
+  PowRef pref;  // the face at this slot, every stage
+  __syncthreads();  // also publishes the flags
+#pragma unroll
+  for (int stage = 0; stage < 3; ++stage) {
+    const int e = 2 + 2 * stage;
+    bool ok = true;
+    if (colok && j >= e && j <= NR - e && k >= 0 && k <= nz) {
+      double v4[4], d3[4], G[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v)
+        recon4(s[v][u - 2 * CW], s[v][u - CW], s[v][u], s[v][u + CW], v4[v], d3[v]);
+      flux_z_face(v4, d3, hz[0][j], hz[1][j], hz[2][j], a.hv, k == 0 || k == nz, G,
+                  stage == 0 ? POW_FIRST : POW_TRY, &pref, &ok);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) fl[v][u] = G[v];
+    }
+    if (!ok) {  // the face again with pow_pos (rare)
+      double v4[4], d3[4], G[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v)
+        recon4(s[v][u - 2 * CW], s[v][u - CW], s[v][u], s[v][u + CW], v4[v], d3[v]);
+      flux_z_face(v4, d3, hz[0][j], hz[1][j], hz[2][j], a.hv, k == 0 || k == nz, G);
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) fl[v][u] = G[v];
+    }
+    wave_phase_done(sm.wf, 2 * stage + 1, t);
+    const double dts = stage == 0 ? a.dt1 : stage == 1 ? a.dt2 : a.dt3;
+    if (colok && j >= e && j <= NR - e - 1 && k >= 0 && k < nz) {
+      double td[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) td[v] = tend_of(fl[v][u], fl[v][u + CW], a.d, id);
+      td[2] = td[2] - s[0][u] * kGrav;
+      if (a.src_w) td[2] = td[2] + src;
+      double q[4];
+#pragma unroll
+      for (int v = 0; v < kNv; ++v) q[v] = S[v] + dts * td[v];
+      if (stage < 2) {
+#pragma unroll
+        for (int v = 0; v < kNv; ++v) s[v][u] = q[v];  // in place: faces are done
+        // the wall halo rows of the new stage state (slots u -+ CW, 2 CW of
+        // this column group; no thread reads them in this phase)
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+          if (k == (side == 0 ? 0 : nz - 1)) {
+            const int pe = side == 0 ? plo : phi;
+            const int step = side == 0 ? -CW : CW;
+#pragma unroll
+            for (int h = 1; h <= 2; ++h) {
+              const int kh = side == 0 ? -h : nz - 1 + h;  // the halo row's k
+              double w[4];
+#pragma unroll
+              for (int v = 0; v < kNv; ++v) w[v] = wall_halo(v, q[v], hy[pe], hy[kh + kHs]);
+#pragma unroll
+              for (int v = 0; v < kNv; ++v) s[v][u + h * step] = w[v];
+              if (stage == 1 && own) {  // set_bc_z of stage 3
+#pragma unroll
+                for (int v = 0; v < kNv; ++v) a.tmp[g.at(v, kh + kHs, pc)] = w[v];
+              }
+            }
+          }
+        }
+      }
+      if (own) {
+        gdouble* dst = stage == 0 ? a.s1 : stage == 1 ? tmp_own : a.out;
+        if (dst) {
+#pragma unroll
+          for (int v = 0; v < kNv; ++v) dst[g.at(v, k + kHs, pc)] = q[v];
+        }
+      }
+    }
+    if (stage == 2) break;
+    wave_phase_done(sm.wf, 2 * stage + 2, t);
+  }
+}
+
 template <int CW, int NR>
 __global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_dir_zt2d(DirArgs a, Grid g, int nreal,
                                                           int per_xcd) {
@@ -1233,6 +1388,16 @@ __global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))
   member_offset(a);
   dir_zt2_tile<CW, NR>(a, g, (logical % ncolgrp) * CW, (logical / ncolgrp) * (NR - 12), sm,
                        logical == 0);
+}
+
+template <int CW, int NR>
+__global__ __launch_bounds__(CW * NR) __attribute__((amdgpu_waves_per_eu(6))) void k_dir_zt1d(
+    DirArgs a, Grid g, int nreal, int per_xcd) {
+  __shared__ ZTile2HLds<CW, NR> sm;
+  const int logical = xcd_logical(blockIdx.x, per_xcd);
+  if (logical >= nreal) return;  // whole block, before any barrier
+  member_offset(a);
+  dir_zt1d_tile<CW, NR>(a, g, logical, sm, logical == 0);
 }
 
 // The same tiles with one cell per thread (CW x NR threads): at 8 x 96, two
@@ -1385,6 +1550,19 @@ __global__ __launch_bounds__(CW * NR / 2) __attribute__((amdgpu_waves_per_eu(4))
 }
 
 template <int CW, int NR>
+__global__ __launch_bounds__(CW * NR) __attribute__((amdgpu_waves_per_eu(6))) void k_ens_zt1d(
+    const MemberDev* __restrict__ m, int n, int pass, int total, int per_xcd) {
+  __shared__ ZTile2HLds<CW, NR> sm;
+  const int item = xcd_logical(blockIdx.x, per_xcd);
+  if (item >= total) return;  // whole block, before any barrier
+  const MemberDev& md = m[member_of<1>(m, n, item)];
+  const DirArgs a = member_args(md, 1, pass & 1, pass >> 1);
+  const int local = item - md.z_start;
+  if (local >= md.z_ncol) return;  // dense z: z_ncol holds the member's blocks
+  dir_zt1d_tile<CW, NR>(a, Grid(md.nx, md.nz), local, sm, local == 0);
+}
+
+template <int CW, int NR>
 __global__ __launch_bounds__(CW * NR) __attribute__((amdgpu_waves_per_eu(6))) void k_ens_zt1(
     const MemberDev* __restrict__ m, int n, int pass, int total, int per_xcd) {
   __shared__ ZTile2HLds<CW, NR> sm;
@@ -1432,6 +1610,15 @@ int launch_zt1(const DirArgs& a, Grid g, hipStream_t st, int members) {
   const int per = (int)((nreal + kNumXcd - 1) / kNumXcd);
   k_dir_zt1<CW, NR><<<dim3(per * kNumXcd, members), CW * NR, 0, st>>>(a, g, ncolgrp,
                                                                       (int)nreal, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
+template <int CW, int NR>
+int launch_zt1d(const DirArgs& a, Grid g, hipStream_t st, int members) {
+  const int nreal = dense_z_blocks(g.nx, g.nz, CW, NR);
+  const int per = (nreal + kNumXcd - 1) / kNumXcd;
+  k_dir_zt1d<CW, NR><<<dim3(per * kNumXcd, members), CW * NR, 0, st>>>(a, g, nreal, per);
   PMW_CHECK_LAUNCH();
   return PMW_OK;
 }
@@ -1502,7 +1689,7 @@ bool fused_x_tile_ok(int nt) {
 // cells per thread); 18128 = 8128 as dense tiles (column groups end to end)
 bool fused_z_seg_ok(int seg) {
   return seg == 8 || seg == 16 || seg == 24 || seg == 32 || seg == 48 || seg == 64 ||
-         seg == 8064 || seg == 4128 || seg == 8128 || seg == 18128 || seg == 8096;
+         seg == 8064 || seg == 4128 || seg == 8128 || seg == 18128 || seg == 8096 || seg == 18096;
 }
 
 int launch_dir_x(const DirArgs& a, Grid g, int nt, hipStream_t st, int members) {
@@ -1534,6 +1721,9 @@ int launch_dir_z(const DirArgs& a, Grid g, int seg, hipStream_t st, int members)
     case 18128:
       if (dense_z_rows(g.nx, g.nz, 8) > (1LL << 30)) return PMW_EINVAL;
       return launch_zt2d<8, 128>(a, g, st, members);
+    case 18096:
+      if (dense_z_rows(g.nx, g.nz, 8) > (1LL << 30)) return PMW_EINVAL;
+      return launch_zt1d<8, 96>(a, g, st, members);
     default: return PMW_EINVAL;
   }
 }
@@ -1565,9 +1755,9 @@ int plan_members(const pmw_member* mem, int n, int x_nt, int z_seg, MemberDev* d
     const int xt = dense ? dense_x_blocks(m.nx, m.nz, x_nt - 1000)
                          : (m.nx + ens_x_cells(x_nt) - 1) / ens_x_cells(x_nt);
     int zc, zsg;
-    if (z_seg == 18128) {  // dense tiles: zc = the member's blocks, one "segment"
+    if (z_seg == 18128 || z_seg == 18096) {  // dense tiles: zc = the member's blocks, one "segment"
       if (dense_z_rows(m.nx, m.nz, 8) > (1LL << 30)) return PMW_EINVAL;
-      zc = dense_z_blocks(m.nx, m.nz, 8, 128);
+      zc = dense_z_blocks(m.nx, m.nz, 8, z_seg % 1000);
       zsg = 1;
     } else if (z_seg >= 1000) {  // LDS tiles: columns*1000 + rows (12 of them halo)
       const int cw = z_seg / 1000, rows = z_seg % 1000 - 12;
@@ -1692,6 +1882,14 @@ int launch_ens_zt2d(const MemberDev* d, int n, int pass, int total, hipStream_t 
 }
 
 template <int CW, int NR>
+int launch_ens_zt1d(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
+  const int per = ens_per_xcd(total);
+  k_ens_zt1d<CW, NR><<<per * kNumXcd, CW * NR, 0, st>>>(d, n, pass, total, per);
+  PMW_CHECK_LAUNCH();
+  return PMW_OK;
+}
+
+template <int CW, int NR>
 int launch_ens_zt1(const MemberDev* d, int n, int pass, int total, hipStream_t st) {
   const int per = ens_per_xcd(total);
   k_ens_zt1<CW, NR><<<per * kNumXcd, CW * NR, 0, st>>>(d, n, pass, total, per);
@@ -1735,6 +1933,7 @@ int launch_ens(int dir, int x_nt, int z_seg, const MemberDev* d, int n, int pass
     case 8128: return launch_ens_zt2<8, 128>(d, n, pass, plan.z_total, st);
     case 8096: return launch_ens_zt1<8, 96>(d, n, pass, plan.z_total, st);
     case 18128: return launch_ens_zt2d<8, 128>(d, n, pass, plan.z_total, st);
+    case 18096: return launch_ens_zt1d<8, 96>(d, n, pass, plan.z_total, st);
     default: return launch_ens_z<64>(d, n, pass, plan.z_total, st);
   }
 }

@@ -12,6 +12,7 @@ import bench  # noqa: E402
 def test_fused_kernel_names_follow_the_tuning_codes():
     assert bench.fused_kernel_names((0, 512, 8128)) == ("k_dir_x<512>", "k_dir_zt2<8, 128>")
     assert bench.fused_kernel_names((0, 1256, 18128)) == ("k_dir_xd<256>", "k_dir_zt2d<8, 128>")
+    assert bench.fused_kernel_names((0, 1256, 18096)) == ("k_dir_xd<256>", "k_dir_zt1d<8, 96>")
     assert bench.fused_kernel_names((0, 512, 8096)) == ("k_dir_x<512>", "k_dir_zt1<8, 96>")
     assert bench.fused_kernel_names((0, 256, 4128)) == ("k_dir_x<256>", "k_dir_zt<4, 512>")
     assert bench.fused_kernel_names((0, 448, 8064)) == ("k_dir_x<448>", "k_dir_zt<8, 512>")

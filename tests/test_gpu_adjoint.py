@@ -162,7 +162,8 @@ def test_gradient_matches_torch_autograd(ic, nx, nz, steps):
     assert r <= 1e-11, f"gradient rel L2 {r:.3e}"
 
 
-@pytest.mark.parametrize("tape_tuning", [None, (0, 1256, 18128), (0, 1512, 8064)])
+@pytest.mark.parametrize("tape_tuning", [None, (0, 1256, 18128), (0, 1512, 8064),
+                                         (0, 1256, 18096)])
 def test_tape_forward_bitwise_equals_evolve(tape_tuning):
     """Every state slot the tape keeps equals the state the fused evolve
     passes through (two members, both direction orders), whatever kernel

@@ -30,7 +30,8 @@ def _host(t):
 
 @pytest.mark.parametrize("tuning", [(0, 256, 16), (0, 512, 8), (0, 256, 32), (0, 512, 8064),
                                     (0, 512, 8128), (0, 256, 4128), (0, 1256, 8128),
-                                    (0, 1512, 24), (0, 1256, 18128), (0, 1256, 8096)])
+                                    (0, 1512, 24), (0, 1256, 18128), (0, 1256, 8096),
+                                    (0, 1256, 18096)])
 def test_members_bitwise_equal_single_grid_solves(tuning):
     import pyminiweather_amd.solve as S
     from pyminiweather_amd.data import initialize_fields
@@ -236,7 +237,7 @@ def test_ensemble_rejects_bad_tuning_and_injection():
         Ensemble([member_params(20, 10, "injection")])
 
 
-@pytest.mark.parametrize("tuning", [(0, 256, 8128), (0, 1256, 18128)])
+@pytest.mark.parametrize("tuning", [(0, 256, 8128), (0, 1256, 18128), (0, 1256, 18096)])
 def test_full_256_member_ensemble(tuning):
     """The seeded 256-member mixed-size ensemble at full size, 6 steps, with
     row tiles and with dense tiles: every member conserves mass to 1e-13

@@ -433,7 +433,7 @@ def _evolve_ex(p, f, nsteps, tuning, flag=0):
 
 FUSED_TUNINGS = [(0, 256, 8), (0, 256, 16), (0, 512, 24), (0, 512, 32), (0, 512, 8064),
                  (0, 256, 8128), (0, 448, 4128), (0, 1256, 8128), (0, 1512, 48), (0, 512, 8096),
-                 (0, 512, 18128), (0, 1256, 18128)]
+                 (0, 512, 18128), (0, 1256, 18128), (0, 512, 18096), (0, 1256, 18096)]
 
 
 @pytest.mark.parametrize("ic,nx,nz", [("collision", 60, 30), ("gravity", 40, 20),
@@ -474,7 +474,7 @@ def test_run_equals_evolve_loop_bitwise():
     np.testing.assert_array_equal(host(f1.state_tmp), host(f2.state_tmp))
 
 
-@pytest.mark.parametrize("tuning", [(0, 512, 8128), (0, 1256, 18128)])
+@pytest.mark.parametrize("tuning", [(0, 512, 8128), (0, 1256, 18128), (0, 512, 18096)])
 def test_large_grid_fused_equals_staged_and_conserves_mass(tuning):
     """8192 x 4096 (3.4e7 cells, 1.1 GB per state; 17x the 2000 x 1000
     workload): tile and block counts and 64-bit offsets far past the
