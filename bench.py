@@ -185,7 +185,7 @@ def bench_ensemble(nsteps, loop_steps, autotune):
         per_size[f"{key[0]}x{key[1]}"] = {"members": len(group),
                                           "cell_steps_per_s": cells_g * loop_steps / dt_g}
     loop_rate = ens.cells * loop_steps / t_loop
-    # the reference's driver shape: one evolve() call per member and step
+    # the reference's main-loop shape: one evolve() call per member and step
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for p, f, mesh in zip(ens.params, ens.fields, ens.meshes):

@@ -65,7 +65,7 @@ __global__ void k_halo_x(double* __restrict__ s, Grid g, int bc,
 // Right x-halo columns nx+2, nx+3 of the interior rows, all 4 variables,
 // from `src` to `dst`. With the injection inflow set_bc_x leaves that halo as
 // the forcing buffer holds it (bcs.py:37), and the reference's stage 2 reads
-// state_tmp's (step.py:122-131); this build's stage 2 reads state_work, so
+// state_tmp's (step.py:122-131); the stage 2 here reads state_work, so
 // state_tmp's right halo is copied there first.
 __global__ void k_copy_right_halo(const double* __restrict__ src,
                                   double* __restrict__ dst, Grid g) {
