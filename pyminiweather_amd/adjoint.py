@@ -14,7 +14,8 @@ segment s runs on the caller's stream (events order each adjoint after its
 recompute, and each recompute after the adjoint that last read its tape).
 Memory is O(sqrt(N)) states plus two segments' tapes (6 states per step);
 the recompute is one extra forward run. With ``keep_tapes`` (by default as
-many as fit in half of the device memory free when the solver is made) the
+many as fit in three quarters of the device memory free when the solver is
+made, after its two recompute tapes) the
 forward sweep writes the full tapes of its last segments as it goes, and
 the backward sweep uses them without recomputing: HBM capacity traded for
 the recompute (the forward sweep pays the stage-state stores on those
@@ -162,7 +163,7 @@ class AdjointSolver:
         tape_bytes = 8 * math.prod(tshape)
         if keep_tapes == "auto":
             free, _ = torch.cuda.mem_get_info(dev)
-            keep = int((free // 2) // tape_bytes)
+            keep = int((free * 3 // 4) // tape_bytes)
         else:
             keep = int(keep_tapes)
         keep = max(0, min(keep, nseg))
