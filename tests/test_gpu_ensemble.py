@@ -171,13 +171,15 @@ def test_config_size_members_match_oracle_after_300_steps():
         print(f"{t} member {idx[j]} after 300 steps: " + "; ".join(line), flush=True)
 
 
+@pytest.mark.parametrize("tuning", [(0, 256, 16), (0, 1256, 18096), (0, 512, 8096)])
 @pytest.mark.parametrize("pairs", [1, 3])
-def test_graph_replays_match_eager(pairs):
+def test_graph_replays_match_eager(pairs, tuning):
     """Each replay of a captured 2- or 6-step graph, with eager work and host
     syncs between replays, matches a twin stepped eagerly one step per call
-    bit for bit."""
-    a = _ensemble()
-    b = _ensemble()
+    bit for bit (marching z kernel, and the one-cell z tiles whose waves
+    order their phases by LDS flags)."""
+    a = _ensemble(tuning)
+    b = _ensemble(tuning)
     g = a.capture(step_pairs=pairs)
     # capture records but does not run: states are unchanged
     np.testing.assert_array_equal(_host(a.fields[0].state), _host(b.fields[0].state))
