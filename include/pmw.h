@@ -300,10 +300,11 @@ long long pmw_tape_slot_len(int nx, int nz, int members);
 /* nsteps evolve steps (pyminiweather/solve/step.py:85-143, periodic x) of
  * `members` identical grids that share the hydrostatic arrays and src_w,
  * one fused launch per direction for all members. Slot 0 of `tape` holds
- * the start states; step k writes slots 6k+1 .. 6k+6: stage 1 (only with
- * store_stages) and stage 2 of its first direction, the state after it,
- * stage 1 (with store_stages) and stage 2 of the second direction, and the
- * state after the step. reverse_direction is the reference toggle at step
+ * the start states; with store_stages, step k writes slots 6k+1 .. 6k+6:
+ * stage 1 and stage 2 of its first direction, the state after it, stage 1
+ * and stage 2 of the second direction, and the state after the step.
+ * Without, only the end states are kept, in slot 6*nsteps (slots 0..2 serve
+ * as scratch on the way). reverse_direction is the reference toggle at step
  * 0 (nonzero = x first); it alternates per step. tape_len >= (6*nsteps+1)
  * slots. tuning as pmw_evolve_ex (fused mode only; NULL = default). The
  * interiors equal pmw_evolve_ex's bit for bit. No reference counterpart. */

@@ -720,12 +720,27 @@ int pmw_tape_forward(double* tape, long long tape_len, int members,
     const int first = rev ? 0 : 1;  // 0 = x, 1 = z
     for (int pass = 0; pass < 2; ++pass) {
       const int dir = pass == 0 ? first : 1 - first;
-      double* base = tape + (6LL * k + 3 * pass) * slot;
-      a.in = gptr(base);
-      a.st = gptr(base);  // halo refresh of the start state (never read by this launch)
-      a.s1 = store_stages ? gptr(base + slot) : nullptr;
-      a.tmp = gptr(base + 2 * slot);
-      a.out = gptr(base + 3 * slot);
+      if (store_stages) {
+        double* base = tape + (6LL * k + 3 * pass) * slot;
+        a.in = gptr(base);
+        a.st = gptr(base);  // halo refresh of the start state (never read by this launch)
+        a.s1 = gptr(base + slot);
+        a.tmp = gptr(base + 2 * slot);
+        a.out = gptr(base + 3 * slot);
+        a.tmp_interior = true;
+      } else {
+        // end state only: the states alternate between slots 0 and 1 (a
+        // working set the size of evolve's buffers instead of a stream of
+        // fresh slots), slot 2 takes the stage-3 halo refreshes without the
+        // dead stage-2 interior, and the last direction writes slot 6n
+        const long long q = 2LL * k + pass;
+        a.st = gptr(tape + (q % 2) * slot);  // halo refresh of the start state
+        a.in = a.st;
+        a.s1 = nullptr;
+        a.tmp = gptr(tape + 2 * slot);
+        a.out = gptr(tape + (q == 2LL * nsteps - 1 ? 6LL * nsteps : (q + 1) % 2) * slot);
+        a.tmp_interior = false;
+      }
       a.d = dir == 0 ? dx : dz;
       a.id = 1.0 / a.d;
       a.hv = dir == 0 ? hv_coeff_x : hv_coeff_z;
