@@ -36,6 +36,11 @@
 
 using namespace pmw;
 
+// The adjoint needs no bit-for-bit agreement with the reference's
+// arithmetic (only with itself across kernel shapes, which share this
+// code): its expressions may contract to fma.
+#pragma clang fp contract(fast)
+
 namespace {
 
 constexpr double kW4[4] = {kW4a, kW4b, kW4b, kW4a};
