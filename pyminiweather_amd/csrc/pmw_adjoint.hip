@@ -43,6 +43,12 @@ using namespace pmw;
 
 namespace {
 
+// The 4-tap face value alone (the adjoint's faces need no third
+// difference), symmetric taps summed first.
+__device__ __forceinline__ double recon_val(double s0, double s1, double s2, double s3) {
+  return (s0 + s3) * kW4a + (s1 + s2) * kW4b;
+}
+
 constexpr double kW4[4] = {kW4a, kW4b, kW4b, kW4a};
 constexpr double kD3[4] = {-1.0, 3.0, -3.0, 1.0};
 constexpr int kAdjXThreads = 256;          // x: faces per block
@@ -186,13 +192,13 @@ __global__ __launch_bounds__(kAdjXThreads) void k_adj_x(AdjArgs a, Grid g, int n
       const int c = face - 2 + j;
       col[j] = (c >= 0 && c < nx ? c : wrapc(c, nx)) + kHs;
     }
-    double val[4], d3[4], lf[4], lv[4], ld[4];
+    double val[4], lf[4], lv[4], ld[4];
 #pragma unroll
     for (int v = 0; v < kNv; ++v) {
       double s[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) s[j] = a.f[g.at(v, prow, col[j])];
-      recon4(s[0], s[1], s[2], s[3], val[v], d3[v]);
+      val[v] = recon_val(s[0], s[1], s[2], s[3]);
     }
     // T(i) = (F(i) - F(i+1)) / dx: the face is F(i) of cell `face` (col[2])
     // and F(i+1) of cell face-1 (col[1])
@@ -260,13 +266,13 @@ __global__ __launch_bounds__(64 * kAdjZWaves) void k_adj_z(AdjArgs a, Grid g, in
     for (int s = wy; s < NF; s += kAdjZWaves) {
       const int face = r0 - 1 + s;
       if (face < 0 || face > nz) continue;
-      double val[4], d3[4], lf[4], lv[4], ld[4];
+      double val[4], lf[4], lv[4], ld[4];
 #pragma unroll
       for (int v = 0; v < kNv; ++v) {
         double q[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) q[j] = zrow_value(a.f, g, a.hy_cell, v, face - 2 + j, pc);
-        recon4(q[0], q[1], q[2], q[3], val[v], d3[v]);
+        val[v] = recon_val(q[0], q[1], q[2], q[3]);
       }
 #pragma unroll
       for (int v = 0; v < kNv; ++v) {
@@ -454,10 +460,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 #pragma unroll
       for (int q = 0; q < kNv; ++q) sm.fw[q][t] = 0.0;
     } else if (j >= 2 + 2 * m && j <= NR - 2 - 2 * m && (DIR == 0 || col + (blk % nline) * CW < nx)) {
-      double val[4], d3[4], lf[4], lv[4], ld[4];
+      double val[4], lf[4], lv[4], ld[4];
 #pragma unroll
       for (int v = 0; v < kNv; ++v)
-        recon4(sm.st[v][t - 2 * S], sm.st[v][t - S], sm.st[v][t], sm.st[v][t + S], val[v], d3[v]);
+        val[v] = recon_val(sm.st[v][t - 2 * S], sm.st[v][t - S], sm.st[v][t], sm.st[v][t + S]);
 #pragma unroll
       for (int v = 0; v < kNv; ++v) lf[v] = sd * (lin[v][t] - lin[v][t - S]);
       // the face's power: evaluated at stage 3, continued at stages 2 and 1
@@ -478,10 +484,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
       for (int v = 0; v < kNv; ++v) sm.fw[v][t] = lv[v];
     }
     if (!pok) {  // the face again with the power itself (rare)
-      double val[4], d3[4], lf[4], lv[4], ld[4];
+      double val[4], lf[4], lv[4], ld[4];
 #pragma unroll
       for (int v = 0; v < kNv; ++v)
-        recon4(sm.st[v][t - 2 * S], sm.st[v][t - S], sm.st[v][t], sm.st[v][t + S], val[v], d3[v]);
+        val[v] = recon_val(sm.st[v][t - 2 * S], sm.st[v][t - S], sm.st[v][t], sm.st[v][t + S]);
 #pragma unroll
       for (int v = 0; v < kNv; ++v) lf[v] = sd * (lin[v][t] - lin[v][t - S]);
       const double pw = dpow(val[3] + hy_t);
