@@ -474,7 +474,8 @@ def test_run_equals_evolve_loop_bitwise():
     np.testing.assert_array_equal(host(f1.state_tmp), host(f2.state_tmp))
 
 
-@pytest.mark.parametrize("tuning", [(0, 512, 8128), (0, 1256, 18128), (0, 512, 18096)])
+@pytest.mark.parametrize("tuning", [(0, 512, 8128), (0, 1256, 18128), (0, 512, 18096),
+                                    (0, 512, 8096)])
 def test_large_grid_fused_equals_staged_and_conserves_mass(tuning):
     """8192 x 4096 (3.4e7 cells, 1.1 GB per state; 17x the 2000 x 1000
     workload): tile and block counts and 64-bit offsets far past the
