@@ -199,6 +199,24 @@ def test_tape_forward_bitwise_equals_evolve(tape_tuning):
                                           st[:, 2:-2, 2:-2].cpu().numpy())
 
 
+@pytest.mark.parametrize("nsteps", [1, 2, 5])
+def test_checkpoint_only_tape_forward_equals_full_tape(nsteps):
+    """Without stage stores the tape forward keeps only the end states (the
+    states alternate between scratch slots): they equal the full tape's end
+    states bit for bit, for odd and even step counts and two members."""
+    from pyminiweather_amd.adjoint import AdjointSolver
+    p = O.make_params(70, 33, "collision", cfl=1.5, max_speed=450.0)
+    f = _fields(p)
+    solver = AdjointSolver(p, f, nsteps, checkpoint_every=nsteps, members=2, keep_tapes=0)
+    s0 = torch.stack([f.state, f.state.clone()])
+    s0[1, :, 2:-2, 2:-2] += _perturbed_interior(f, 5) - f.state[:, 2:-2, 2:-2]
+    end_full = solver._tape_run(s0, 1, nsteps, store=True).clone()
+    end_ckpt = solver._tape_run(s0, 1, nsteps, store=False).clone()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(end_ckpt[:, :, 2:-2, 2:-2].cpu().numpy(),
+                                  end_full[:, :, 2:-2, 2:-2].cpu().numpy())
+
+
 def test_tape_stage1_slots_match_rk_stage():
     """Stage-1 slots hold S + dt/3 T(S) as the per-stage kernel computes it."""
     from pyminiweather_amd import _lib
